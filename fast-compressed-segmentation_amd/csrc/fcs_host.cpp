@@ -1,0 +1,715 @@
+// Host runtime of the C ABI (include/fcs/fcs.h): one HIP stream per context,
+// device-resident volumes, a brick cache pool with per-size free lists, and
+// batched decode / march launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fcs/fcs.h"
+#include "fcs_common.h"
+#include "fcs_device.h"
+#include "fcs_kernels.h"
+
+struct fcs_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+  fcs_timing timing{};
+};
+
+namespace {
+
+struct Alloc {
+  uint64_t off = 0, size = 0;
+};
+
+}  // namespace
+
+struct fcs_volume {
+  fcs_ctx* ctx = nullptr;
+  uint32_t nb = 0, b = 0, L = 0, B = 0, label_bits = 32, header_b = 0;
+  uint64_t compressed = 0, symbols = 0, palette_entries = 0;
+  uint32_t freq[16]{}, cum[17]{};
+  std::vector<DevBrick> hb;
+  std::vector<uint8_t> empty;     // all palette entries are 0
+  std::vector<uint64_t> pal0;     // palette[0] per brick
+  std::vector<uint8_t> levels_ok; // from the rANS stage
+  std::vector<uint32_t> rans_status;
+  // device
+  uint8_t* d_stream = nullptr;
+  DevBrick* d_bricks = nullptr;
+  void* d_pal = nullptr;
+  void* d_pal0 = nullptr;
+  uint32_t* d_nib = nullptr;
+  LevelEntry* d_lvl = nullptr;
+  RansResult* d_rres = nullptr;
+  uint32_t* d_counter = nullptr;
+  bool rans_ready = false;
+  // brick cache pool (voxels of label_bits/8 bytes)
+  void* d_cache = nullptr;
+  uint64_t cap = 0, bump = 0;
+  std::vector<std::vector<uint64_t>> free_lists;  // by size class
+  std::vector<int16_t> res_lod;   // -1 = not resident
+  std::vector<uint8_t> res_pyr;   // 1 = pyramid layout (levels 0..res_lod)
+  std::vector<Alloc> res_alloc;
+  // staging
+  DecodeReq* d_req = nullptr;
+  uint32_t* d_req_status = nullptr;
+  size_t req_cap = 0;
+  uint32_t* d_meta = nullptr;
+  uint64_t* d_offs = nullptr;
+  void* d_flabel = nullptr;
+  uint64_t* d_fvoxel = nullptr;
+  float* d_ft = nullptr;
+  uint32_t* d_visited = nullptr;
+  uint32_t* d_misses = nullptr;
+  size_t frame_cap = 0;
+
+  size_t lbytes() const { return label_bits / 8; }
+};
+
+namespace {
+
+int fail(fcs_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((ctx), FCS_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));   \
+  } while (0)
+
+uint32_t rd32(const uint8_t* p) {
+  uint32_t v;
+  memcpy(&v, p, 4);
+  return v;
+}
+
+int size_class(uint64_t voxels) {
+  // sizes are 8^t or pyramid sizes (8^(t+1)-1)/7: class = (t, pyramid)
+  for (int t = 0; t <= 6; ++t) {
+    if (voxels == (1ull << (3 * t))) return t;
+    if (voxels == fcs_pyramid_level_offset(t + 1)) return 8 + t;
+  }
+  return -1;
+}
+
+bool cache_alloc(fcs_volume* v, uint64_t voxels, uint64_t* off) {
+  const int cls = size_class(voxels);
+  if (cls >= 0 && (size_t)cls < v->free_lists.size() && !v->free_lists[cls].empty()) {
+    *off = v->free_lists[cls].back();
+    v->free_lists[cls].pop_back();
+    return true;
+  }
+  // keep every allocation 16-voxel aligned for vector stores
+  const uint64_t start = (v->bump + 15) & ~15ull;
+  if (start + voxels > v->cap) return false;
+  *off = start;
+  v->bump = start + voxels;
+  return true;
+}
+
+void cache_release(fcs_volume* v, uint32_t brick) {
+  if (v->res_lod[brick] < 0) return;
+  const Alloc a = v->res_alloc[brick];
+  const int cls = size_class(a.size);
+  if (cls >= 0) v->free_lists[cls].push_back(a.off);
+  v->res_lod[brick] = -1;
+}
+
+int ensure_req(fcs_volume* v, size_t n) {
+  if (n <= v->req_cap) return 0;
+  fcs_ctx* c = v->ctx;
+  if (v->d_req) { (void)hipFree(v->d_req); v->d_req = nullptr; }
+  if (v->d_req_status) { (void)hipFree(v->d_req_status); v->d_req_status = nullptr; }
+  size_t cap = std::max<size_t>(n, 1024);
+  HIP_TRY(c, hipMalloc(&v->d_req, cap * sizeof(DecodeReq)));
+  HIP_TRY(c, hipMalloc(&v->d_req_status, cap * sizeof(uint32_t)));
+  v->req_cap = cap;
+  return 0;
+}
+
+int ensure_cache(fcs_volume* v) {
+  if (v->d_cache) return 0;
+  return fcs_cache_reserve(v, (uint64_t)v->nb * fcs_pyramid_level_offset(v->L + 1));
+}
+
+// Launch the op decoder over `reqs`; statuses copied back.
+int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
+               uint64_t* voxels_out) {
+  fcs_ctx* c = v->ctx;
+  status.assign(reqs.size(), 0);
+  if (reqs.empty()) {
+    c->timing.decode_ms = 0.0f;
+    return 0;
+  }
+  if (int rc = ensure_req(v, reqs.size())) return rc;
+  HIP_TRY(c, hipMemcpyAsync(v->d_req, reqs.data(), reqs.size() * sizeof(DecodeReq), hipMemcpyHostToDevice,
+                            c->stream));
+  const uint32_t grid = (uint32_t)std::min<size_t>(reqs.size(), (size_t)c->num_cu * 2);
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (v->label_bits == 32) {
+    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, v->d_lvl, v->d_rres, (const uint32_t*)v->d_pal, v->d_req,
+                        (uint32_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, grid, c->stream));
+  } else {
+    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, v->d_lvl, v->d_rres, (const uint64_t*)v->d_pal, v->d_req,
+                        (uint64_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, grid, c->stream));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  HIP_TRY(c, hipMemcpyAsync(status.data(), v->d_req_status, reqs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->timing.decode_ms = ms;
+  uint64_t vox = 0, bytes = 0;
+  for (const auto& q : reqs) {
+    const uint32_t t = q.lod & 0xFF;
+    const uint64_t n = (q.lod & FCS_REQ_PYRAMID) ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
+    vox += n;
+    const DevBrick& bk = v->hb[q.brick < v->nb ? q.brick : 0];
+    // algorithmic bytes: nibble stream consumed (0.5 B/nibble) + palette +
+    // 48 B of brick records + labels written
+    bytes += (uint64_t)(bk.nsym + 1) / 2 + 8ull * bk.P + 48 + n * v->lbytes();
+  }
+  c->timing.decoded_voxels = vox;
+  c->timing.decode_bytes = bytes;
+  if (voxels_out) *voxels_out = vox;
+  return 0;
+}
+
+template <typename T>
+int dev_upload(fcs_ctx* c, T** dst, const void* src, size_t bytes) {
+  HIP_TRY(c, hipMalloc((void**)dst, bytes ? bytes : 16));
+  if (bytes) HIP_TRY(c, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcs_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int fcs_create(int hip_device, fcs_ctx** out) {
+  if (!out) return FCS_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FCS_E_NODEVICE;
+  if (hip_device < 0 || hip_device >= n) return FCS_E_INVALID;
+  fcs_ctx* c = new fcs_ctx();
+  c->device = hip_device;
+  if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FCS_E_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return 0;
+}
+
+void fcs_destroy(fcs_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* fcs_last_error(const fcs_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int fcs_sync(fcs_ctx* c) {
+  if (!c) return FCS_E_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fcs_get_timing(const fcs_ctx* c, fcs_timing* out) {
+  if (!c || !out) return FCS_E_INVALID;
+  *out = c->timing;
+  return 0;
+}
+
+int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_volume** out) {
+  if (!c || !bytes || !out) return fail(c, FCS_E_INVALID, "null argument");
+  *out = nullptr;
+  if (n < 16 + 8 + 128) return fail(c, FCS_E_FORMAT, "stream too short for a .csdb header");
+  if (rd32(bytes) != FCS_MAGIC) return fail(c, FCS_E_FORMAT, "Invalid file format");
+  HIP_TRY(c, hipSetDevice(c->device));
+  auto v = new fcs_volume();
+  v->ctx = c;
+  v->nb = rd32(bytes + 8);
+  v->header_b = rd32(bytes + 12);
+  v->b = bso ? bso : v->header_b;
+  const uint32_t symbols = rd32(bytes + 16), total = rd32(bytes + 20);
+  auto bad = [&](const std::string& m) {
+    fcs_volume_free(v);
+    return fail(c, FCS_E_FORMAT, m);
+  };
+  v->L = fcs_log2u(v->b);
+  if (v->b < 2 || v->b > 64 || (1u << v->L) != v->b) return bad("brick size must be a power of two in [2, 64]");
+  if (symbols != 16 || total != FCS_RANS_M) return bad("rANS model must have 16 symbols and total 4096");
+  uint32_t acc = 0;
+  for (int i = 0; i < 16; ++i) {
+    v->freq[i] = rd32(bytes + 24 + 4 * i);
+    v->cum[i] = rd32(bytes + 24 + 64 + 4 * i);
+    if (v->cum[i] != acc || v->freq[i] == 0) return bad("inconsistent rANS model");
+    acc += v->freq[i];
+  }
+  if (acc != FCS_RANS_M) return bad("rANS frequencies do not sum to 4096");
+  v->cum[16] = FCS_RANS_M;
+  // brick grid: a full power-of-two cube (bricks are indexed by Morton code)
+  uint32_t B = 1;
+  while ((uint64_t)B * B * B < v->nb) B <<= 1;
+  if ((uint64_t)B * B * B != v->nb || B > 1024) return bad("brick count must be a power-of-two cube");
+  v->B = B;
+  struct Raw { uint32_t id, P, E, nsym; size_t pal, str; };
+  std::vector<Raw> raw(v->nb);
+  size_t p = 16 + 8 + 128;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    if (p + 8 > n) return bad("truncated brick header");
+    Raw r;
+    r.id = rd32(bytes + p);
+    r.P = rd32(bytes + p + 4);
+    p += 8;
+    if (r.P == 0 || p + 8ull * r.P + 4 > n) return bad("truncated palette");
+    r.pal = p;
+    p += 8ull * r.P;
+    r.E = rd32(bytes + p);
+    p += 4;
+    if (p + (uint64_t)r.E + 4 > n) return bad("truncated stream");
+    r.str = p;
+    p += r.E;
+    r.nsym = rd32(bytes + p);
+    p += 4;
+    if (r.id >= v->nb) return bad("brick ID outside the brick grid");
+    raw[i] = r;
+  }
+  std::sort(raw.begin(), raw.end(), [](const Raw& a, const Raw& b) { return a.id < b.id; });
+  for (uint32_t i = 0; i < v->nb; ++i)
+    if (raw[i].id != i) return bad("brick IDs are not a permutation of the grid");
+  // host staging
+  uint64_t str_total = 0, nib_total = 0, pal_total = 0;
+  v->hb.resize(v->nb);
+  v->empty.resize(v->nb);
+  v->pal0.resize(v->nb);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    DevBrick& d = v->hb[i];
+    d.str_off = str_total;
+    d.E = raw[i].E;
+    d.nsym = raw[i].nsym;
+    d.pal_off = (uint32_t)pal_total;
+    d.P = raw[i].P;
+    d.nib_off = nib_total;
+    str_total += (raw[i].E + 15) & ~15ull;
+    nib_total += 4ull * std::max<uint64_t>(1, ((uint64_t)raw[i].nsym + 31) / 32);
+    pal_total += raw[i].P;
+    v->compressed += raw[i].E;
+    v->symbols += raw[i].nsym;
+  }
+  v->palette_entries = pal_total;
+  if (pal_total >= (1ull << 32)) return bad("palette too large");
+  std::vector<uint8_t> hs(str_total + 16, 0);
+  std::vector<uint64_t> hp(pal_total);
+  bool wide = false;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    memcpy(hs.data() + v->hb[i].str_off, bytes + raw[i].str, raw[i].E);
+    memcpy(hp.data() + v->hb[i].pal_off, bytes + raw[i].pal, 8ull * raw[i].P);
+    bool all0 = true;
+    for (uint32_t k = 0; k < raw[i].P; ++k) {
+      const uint64_t x = hp[v->hb[i].pal_off + k];
+      all0 &= (x == 0);
+      wide |= (x >> 32) != 0;
+    }
+    v->empty[i] = all0;
+    v->pal0[i] = hp[v->hb[i].pal_off];
+  }
+  v->label_bits = wide ? 64 : 32;
+  int rc = 0;
+  if ((rc = dev_upload(c, &v->d_stream, hs.data(), hs.size())) ||
+      (rc = dev_upload(c, &v->d_bricks, v->hb.data(), v->hb.size() * sizeof(DevBrick)))) {
+    fcs_volume_free(v);
+    return rc;
+  }
+  if (wide) {
+    rc = dev_upload(c, (uint64_t**)&v->d_pal, hp.data(), hp.size() * 8);
+    if (!rc) rc = dev_upload(c, (uint64_t**)&v->d_pal0, v->pal0.data(), v->pal0.size() * 8);
+  } else {
+    std::vector<uint32_t> h32(hp.begin(), hp.end()), p32(v->pal0.begin(), v->pal0.end());
+    rc = dev_upload(c, (uint32_t**)&v->d_pal, h32.data(), h32.size() * 4);
+    if (!rc) rc = dev_upload(c, (uint32_t**)&v->d_pal0, p32.data(), p32.size() * 4);
+  }
+  if (rc) { fcs_volume_free(v); return rc; }
+  if (hipMalloc(&v->d_nib, (nib_total + 8) * 4) != hipSuccess ||
+      hipMalloc(&v->d_lvl, (size_t)v->nb * (v->L + 1) * sizeof(LevelEntry)) != hipSuccess ||
+      hipMalloc(&v->d_rres, (size_t)v->nb * sizeof(RansResult)) != hipSuccess ||
+      hipMalloc(&v->d_counter, 64) != hipSuccess || hipMalloc(&v->d_meta, (size_t)v->nb * 4) != hipSuccess ||
+      hipMalloc(&v->d_offs, (size_t)v->nb * 8) != hipSuccess) {
+    fcs_volume_free(v);
+    return fail(c, FCS_E_NOMEM, "device allocation failed");
+  }
+  (void)hipMemset(v->d_nib, 0, (nib_total + 8) * 4);
+  v->res_lod.assign(v->nb, -1);
+  v->res_pyr.assign(v->nb, 0);
+  v->res_alloc.assign(v->nb, Alloc{});
+  v->free_lists.assign(16, {});
+  v->levels_ok.assign(v->nb, 0);
+  v->rans_status.assign(v->nb, 0);
+  *out = v;
+  return 0;
+}
+
+void fcs_volume_free(fcs_volume* v) {
+  if (!v) return;
+  if (v->ctx) (void)hipSetDevice(v->ctx->device);
+  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_lvl, v->d_rres, v->d_counter,
+                  v->d_cache, v->d_req, v->d_req_status, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel,
+                  v->d_ft, v->d_visited, v->d_misses};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete v;
+}
+
+int fcs_volume_get_info(const fcs_volume* v, fcs_volume_info* o) {
+  if (!v || !o) return FCS_E_INVALID;
+  memset(o, 0, sizeof *o);
+  o->num_bricks = v->nb;
+  o->brick_size = v->b;
+  o->levels = v->L;
+  o->bricks_per_axis = v->B;
+  o->grid_size = v->B * v->b;
+  o->label_bits = v->label_bits;
+  o->header_brick_size = v->header_b;
+  o->rans_ready = v->rans_ready;
+  o->compressed_bytes = v->compressed;
+  o->total_symbols = v->symbols;
+  o->palette_entries = v->palette_entries;
+  o->cache_capacity_voxels = v->cap;
+  memcpy(o->rans_freq, v->freq, sizeof o->rans_freq);
+  memcpy(o->rans_cum, v->cum, sizeof o->rans_cum);
+  return 0;
+}
+
+int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
+  if (!v) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  RansArgs a{};
+  a.bricks = v->d_bricks;
+  a.stream = v->d_stream;
+  a.list = nullptr;
+  a.nlist = v->nb;
+  a.L = v->L;
+  memcpy(a.freq, v->freq, sizeof a.freq);
+  memcpy(a.cum, v->cum, sizeof a.cum);
+  a.nib = v->d_nib;
+  a.lvl = v->d_lvl;
+  a.res = v->d_rres;
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  HIP_TRY(c, fcs_launch_rans(a, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  std::vector<RansResult> rr(v->nb);
+  HIP_TRY(c, hipMemcpyAsync(rr.data(), v->d_rres, rr.size() * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->timing.rans_ms = ms;
+  c->timing.rans_symbols = v->symbols;
+  c->timing.rans_bytes = v->compressed;
+  bool bad = false;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    v->rans_status[i] = rr[i].status;
+    v->levels_ok[i] = (uint8_t)rr[i].levels_ok;
+    if (out_status) out_status[i] = rr[i].status;
+    bad |= rr[i].status != 0;
+  }
+  v->rans_ready = true;
+  if (bad) return fail(c, FCS_E_CORRUPT, "one or more brick streams failed rANS / structure checks");
+  return 0;
+}
+
+int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
+  if (!v || voxels == 0) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (v->d_cache) { (void)hipFree(v->d_cache); v->d_cache = nullptr; }
+  v->cap = 0;
+  if (hipMalloc(&v->d_cache, voxels * v->lbytes()) != hipSuccess)
+    return fail(c, FCS_E_NOMEM, "brick cache pool allocation failed");
+  v->cap = voxels;
+  return fcs_cache_clear(v);
+}
+
+int fcs_cache_clear(fcs_volume* v) {
+  if (!v) return FCS_E_INVALID;
+  v->bump = 0;
+  for (auto& f : v->free_lists) f.clear();
+  std::fill(v->res_lod.begin(), v->res_lod.end(), (int16_t)-1);
+  std::fill(v->res_pyr.begin(), v->res_pyr.end(), (uint8_t)0);
+  return 0;
+}
+
+int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                       uint64_t* out_off, uint32_t* out_status) {
+  if (!v || (n && (!brick || !lod))) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!v->rans_ready) {
+    int rc = fcs_decode_rans(v, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+  }
+  if (int rc = ensure_cache(v)) return rc;
+  std::vector<DecodeReq> reqs;
+  std::vector<int64_t> req_of(n, -1);
+  std::vector<uint32_t> st(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t bi = brick[i], t = lod[i];
+    if (bi >= v->nb || t > v->L) {
+      st[i] = FCS_ST_BAD_REQUEST;
+      if (out_off) out_off[i] = ~0ull;
+      continue;
+    }
+    if (v->res_lod[bi] >= 0 && ((v->res_pyr[bi] && (uint32_t)v->res_lod[bi] >= t) ||
+                                (!v->res_pyr[bi] && (uint32_t)v->res_lod[bi] == t))) {
+      if (out_off) out_off[i] = v->res_alloc[bi].off + (v->res_pyr[bi] ? fcs_pyramid_level_offset(t) : 0);
+      continue;
+    }
+    cache_release(v, bi);
+    uint64_t off;
+    const uint64_t sz = 1ull << (3 * t);
+    if (!cache_alloc(v, sz, &off)) {
+      st[i] = FCS_ST_CACHE_FULL;
+      if (out_off) out_off[i] = ~0ull;
+      continue;
+    }
+    v->res_lod[bi] = (int16_t)t;
+    v->res_pyr[bi] = 0;
+    v->res_alloc[bi] = Alloc{off, sz};
+    req_of[i] = (int64_t)reqs.size();
+    reqs.push_back(DecodeReq{bi, t, off});
+    if (out_off) out_off[i] = off;
+  }
+  std::vector<uint32_t> rs;
+  if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
+  bool bad = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (req_of[i] >= 0) st[i] |= rs[req_of[i]];
+    bad |= st[i] != 0;
+    if (out_status) out_status[i] = st[i];
+  }
+  if (bad) return fail(c, FCS_E_CORRUPT, "one or more requests failed (see status words)");
+  return 0;
+}
+
+int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
+  if (!v) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!v->rans_ready) {
+    int rc = fcs_decode_rans(v, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+  }
+  const uint32_t t = std::min(lod, v->L);
+  const uint64_t per = pyramid ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
+  if (!v->d_cache || v->cap < per * v->nb) {
+    if (int rc = fcs_cache_reserve(v, per * v->nb)) return rc;
+  }
+  fcs_cache_clear(v);
+  std::vector<DecodeReq> reqs(v->nb);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    const uint64_t off = per * i;
+    reqs[i] = DecodeReq{i, t | (pyramid ? FCS_REQ_PYRAMID : 0u), off};
+    v->res_lod[i] = (int16_t)t;
+    v->res_pyr[i] = pyramid ? 1 : 0;
+    v->res_alloc[i] = Alloc{off, per};
+  }
+  v->bump = per * v->nb;
+  std::vector<uint32_t> rs;
+  if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
+  for (uint32_t s : rs)
+    if (s) return fail(c, FCS_E_CORRUPT, "one or more bricks failed decoding");
+  return 0;
+}
+
+int fcs_read_cache(fcs_volume* v, uint64_t off, uint64_t count, void* host) {
+  if (!v || !host || off + count > v->cap) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(host, (uint8_t*)v->d_cache + off * v->lbytes(), count * v->lbytes(),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fcs_read_brick(fcs_volume* v, uint32_t brick, uint32_t lod, void* host) {
+  if (!v || !host || brick >= v->nb || lod > v->L) return FCS_E_INVALID;
+  const uint8_t t = (uint8_t)lod;
+  uint64_t off = 0;
+  uint32_t st = 0;
+  int rc = fcs_request_bricks(v, &brick, &t, 1, &off, &st);
+  if (rc) return rc;
+  return fcs_read_cache(v, off, 1ull << (3 * lod), host);
+}
+
+int fcs_brick_lods(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint32_t h, uint8_t* out) {
+  if (!v || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  // calculateBrickLOD (camera.js:103-137) with screen width min(w, h)
+  // (webgpuSetup.js:587); focal length from the camera's tan(fov/2)
+  const double sw = (double)std::min(w, h);
+  const double focal = (sw / 2.0) / (double)cam->tan_half_fov;
+  const double half = v->b / 2.0;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    const double cx = fcs_unmorton_x(i) * (double)v->b + half, cy = fcs_unmorton_y(i) * (double)v->b + half,
+                 cz = fcs_unmorton_z(i) * (double)v->b + half;
+    const double dx = cx - cam->position[0], dy = cy - cam->position[1], dz = cz - cam->position[2];
+    double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (d < 0.1) d = 0.1;
+    uint32_t best = 0;
+    for (uint32_t l = 0; l <= v->L; ++l) {
+      const double voxel = (double)v->b / std::pow(2.0, (double)l);
+      if ((focal * voxel) / d >= 1.0) best = l; else break;
+    }
+    out[i] = (uint8_t)best;
+  }
+  return 0;
+}
+
+int fcs_camera_orbit(const fcs_volume* v, double yaw, double pitch, double radius, double fov, fcs_camera* o) {
+  if (!v || !o) return FCS_E_INVALID;
+  // OrbitController.apply (orbitController.js:100-123) around the volume
+  // centre; Camera.getUniformData basis (camera.js:61-77)
+  const double G = (double)v->B * v->b;
+  const double tx = G / 2, ty = G / 2, tz = G / 2;
+  const double cp = std::cos(pitch), sp = std::sin(pitch), cy = std::cos(yaw), sy = std::sin(yaw);
+  const double px = tx + radius * cp * sy, py = ty + radius * sp, pz = tz + radius * cp * cy;
+  auto norm = [](double& x, double& y, double& z) {
+    double l = std::hypot(x, std::hypot(y, z));
+    if (l == 0) l = 1;
+    x /= l; y /= l; z /= l;
+  };
+  double fx = tx - px, fy = ty - py, fz = tz - pz;
+  norm(fx, fy, fz);
+  norm(fx, fy, fz);
+  double ux = 0, uy = 1, uz = 0;
+  if (std::fabs(fy) > 0.999) { ux = 1; uy = 0; }
+  double rx = fy * uz - fz * uy, ry = fz * ux - fx * uz, rz = fx * uy - fy * ux;
+  norm(rx, ry, rz);
+  double vx = ry * fz - rz * fy, vy = rz * fx - rx * fz, vz = rx * fy - ry * fx;
+  norm(vx, vy, vz);
+  memset(o, 0, sizeof *o);
+  o->position[0] = (float)px; o->position[1] = (float)py; o->position[2] = (float)pz;
+  o->tan_half_fov = (float)std::tan(fov * 0.5);
+  o->forward[0] = (float)fx; o->forward[1] = (float)fy; o->forward[2] = (float)fz;
+  o->right[0] = (float)rx; o->right[1] = (float)ry; o->right[2] = (float)rz;
+  o->up[0] = (float)vx; o->up[1] = (float)vy; o->up[2] = (float)vz;
+  return 0;
+}
+
+int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+               fcs_frame* out) {
+  if (!v || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<uint8_t> lods(v->nb);
+  if (brick_lod) {
+    for (uint32_t i = 0; i < v->nb; ++i) lods[i] = std::min<uint8_t>(brick_lod[i], (uint8_t)v->L);
+  } else {
+    fcs_brick_lods(v, cam, w, h, lods.data());
+  }
+  // on-demand decode of every non-empty brick not resident at its LOD
+  std::vector<uint32_t> qb;
+  std::vector<uint8_t> ql;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    if (v->empty[i] || lods[i] == 0) continue;
+    const int r = v->res_lod[i];
+    const bool hit = r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= lods[i]) || (!v->res_pyr[i] && (uint32_t)r == lods[i]));
+    if (!hit) { qb.push_back(i); ql.push_back(lods[i]); }
+  }
+  out->bricks_decoded = (uint32_t)qb.size();
+  out->decode_ms = 0.0f;
+  if (!qb.empty()) {
+    int rc = fcs_request_bricks(v, qb.data(), ql.data(), (uint32_t)qb.size(), nullptr, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+    out->decode_ms = c->timing.decode_ms;
+  }
+  // per-brick frame metadata
+  std::vector<uint32_t> meta(v->nb);
+  std::vector<uint64_t> offs(v->nb, 0);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    uint32_t m = lods[i];
+    if (v->empty[i]) m |= FCS_META_EMPTY;
+    const int r = v->res_lod[i];
+    if (r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= lods[i]) || (!v->res_pyr[i] && (uint32_t)r == lods[i]))) {
+      m |= FCS_META_RESIDENT;
+      offs[i] = v->res_alloc[i].off + (v->res_pyr[i] ? fcs_pyramid_level_offset(lods[i]) : 0);
+    }
+    meta[i] = m;
+    if (out->brick_lod_used) out->brick_lod_used[i] = lods[i];
+  }
+  const size_t npx = (size_t)w * h;
+  if (npx > v->frame_cap) {
+    void* ptrs[] = {v->d_flabel, v->d_fvoxel, v->d_ft};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    HIP_TRY(c, hipMalloc(&v->d_flabel, npx * v->lbytes()));
+    HIP_TRY(c, hipMalloc((void**)&v->d_fvoxel, npx * 8));
+    HIP_TRY(c, hipMalloc((void**)&v->d_ft, npx * 4));
+    v->frame_cap = npx;
+  }
+  if (!v->d_visited) {
+    HIP_TRY(c, hipMalloc((void**)&v->d_visited, (size_t)v->nb * 4));
+    HIP_TRY(c, hipMalloc((void**)&v->d_misses, 64));
+  }
+  HIP_TRY(c, hipMemcpyAsync(v->d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(v->d_offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, fcs_launch_zero(v->d_visited, v->nb, c->stream));
+  HIP_TRY(c, fcs_launch_zero(v->d_misses, 1, c->stream));
+  const float aspect = (float)((double)w / (double)h);
+  MarchCam mc;
+  memcpy(&mc, cam, sizeof mc);
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (v->label_bits == 32) {
+    MarchArgs<uint32_t> a{mc, v->d_meta, v->d_offs, (const uint32_t*)v->d_pal0, (const uint32_t*)v->d_cache,
+                          (uint32_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
+                          w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
+    HIP_TRY(c, fcs_launch_march<uint32_t>(a, c->stream));
+  } else {
+    MarchArgs<uint64_t> a{mc, v->d_meta, v->d_offs, (const uint64_t*)v->d_pal0, (const uint64_t*)v->d_cache,
+                          (uint64_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
+                          w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
+    HIP_TRY(c, fcs_launch_march<uint64_t>(a, c->stream));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  if (out->label) HIP_TRY(c, hipMemcpyAsync(out->label, v->d_flabel, npx * v->lbytes(), hipMemcpyDeviceToHost, c->stream));
+  if (out->voxel) HIP_TRY(c, hipMemcpyAsync(out->voxel, v->d_fvoxel, npx * 8, hipMemcpyDeviceToHost, c->stream));
+  if (out->t_hit) HIP_TRY(c, hipMemcpyAsync(out->t_hit, v->d_ft, npx * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out->brick_visited)
+    HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  out->march_ms = ms;
+  c->timing.march_ms = ms;
+  return 0;
+}
+
+}  // extern "C"

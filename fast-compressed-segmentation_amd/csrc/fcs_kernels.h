@@ -1,0 +1,76 @@
+// Kernel argument blocks and launchers (host-visible declarations).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fcs_device.h"
+
+struct RansArgs {
+  const DevBrick* bricks;
+  const uint8_t* stream;
+  const uint32_t* list;  // nullable: bricks 0..nlist-1
+  uint32_t nlist;
+  uint32_t L;
+  uint32_t freq[16];
+  uint32_t cum[17];
+  uint32_t* nib;
+  LevelEntry* lvl;
+  RansResult* res;
+};
+
+template <typename LT>
+struct OpsArgs {
+  const DevBrick* bricks;
+  const uint32_t* nib;
+  const LevelEntry* lvl;
+  const RansResult* rres;
+  const LT* palette;
+  const DecodeReq* reqs;
+  LT* out;
+  uint32_t* status;
+  uint32_t* counter;  // work counter (reset by the launcher)
+  uint32_t nreq;
+  uint32_t nbricks;
+  uint32_t L;
+};
+
+// per-brick frame metadata for the marcher
+#define FCS_META_LOD_MASK 0xFFu
+#define FCS_META_EMPTY 0x100u
+#define FCS_META_RESIDENT 0x200u
+
+struct MarchCam {
+  float position[3];
+  float tan_half_fov;
+  float forward[3];
+  float ortho_half_height;
+  float right[3];
+  float pad1;
+  float up[3];
+  float pad2;
+};
+
+template <typename LT>
+struct MarchArgs {
+  MarchCam cam;
+  const uint32_t* meta;   // per brick: lod | EMPTY | RESIDENT
+  const uint64_t* offs;   // per brick: cache voxel offset of its labels at `lod`
+  const LT* pal0;         // per brick: palette[0]
+  const LT* cache;
+  LT* out_label;
+  uint64_t* out_voxel;
+  float* out_t;
+  uint32_t* visited;
+  uint32_t* misses;
+  uint32_t w, h;
+  float wf, hf, aspect, grid;
+  uint32_t brick_size, bricks_per_axis;
+};
+
+hipError_t fcs_launch_rans(const RansArgs& a, hipStream_t s);
+size_t fcs_ops_lds_bytes(uint32_t L);
+template <typename LT>
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t grid, hipStream_t s);
+template <typename LT>
+hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s);
+hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s);

@@ -1,0 +1,212 @@
+"""Context / Volume / Frame objects over the C ABI."""
+import ctypes
+import math
+
+import numpy as np
+
+from ._lib import (
+    CameraStruct,
+    FcsError,
+    FrameStruct,
+    TimingStruct,
+    VolumeInfo,
+    c_u8p,
+    c_u32p,
+    c_u64p,
+    c_vp,
+    check,
+    lib,
+)
+
+
+def device_count():
+    return lib.fcs_device_count()
+
+
+def _ptr(a, typ):
+    return a.ctypes.data_as(typ)
+
+
+class Camera:
+    """Camera uniform of camera.js:61-77 plus an orthographic switch."""
+
+    def __init__(self, position, forward, right, up, tan_half_fov=math.tan(math.pi / 6), ortho_half_height=0.0):
+        self.s = CameraStruct()
+        self.s.position[:] = [float(x) for x in position]
+        self.s.forward[:] = [float(x) for x in forward]
+        self.s.right[:] = [float(x) for x in right]
+        self.s.up[:] = [float(x) for x in up]
+        self.s.tan_half_fov = float(tan_half_fov)
+        self.s.ortho_half_height = float(ortho_half_height)
+
+    @classmethod
+    def ortho_plus_z(cls, grid):
+        """Orthographic view along +z covering the whole [0, grid]^2 face."""
+        g = float(grid)
+        return cls((g / 2, g / 2, -1.0), (0, 0, 1), (1, 0, 0), (0, -1, 0), 0.0, g / 2)
+
+    def as_dict(self):
+        s = self.s
+        return {
+            "position": list(s.position), "forward": list(s.forward), "right": list(s.right), "up": list(s.up),
+            "tan_half_fov": s.tan_half_fov, "ortho_half_height": s.ortho_half_height,
+        }
+
+
+class Frame:
+    def __init__(self, label, voxel, t_hit, brick_visited, brick_lod_used, bricks_decoded, decode_ms, march_ms):
+        self.label = label
+        self.voxel = voxel
+        self.t_hit = t_hit
+        self.brick_visited = brick_visited
+        self.brick_lod_used = brick_lod_used
+        self.bricks_decoded = bricks_decoded
+        self.decode_ms = decode_ms
+        self.march_ms = march_ms
+
+
+class Context:
+    """One HIP device + one HIP stream (fcs_create)."""
+
+    def __init__(self, device=0):
+        h = c_vp()
+        rc = lib.fcs_create(device, ctypes.byref(h))
+        if rc < 0:
+            raise FcsError(rc, "fcs_create failed (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.fcs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timing(self):
+        t = TimingStruct()
+        check(lib.fcs_get_timing(self.h, ctypes.byref(t)), self.h)
+        return {f: getattr(t, f) for f, _ in TimingStruct._fields_ if f != "pad"}
+
+    def sync(self):
+        check(lib.fcs_sync(self.h), self.h)
+
+    def load(self, data, brick_size_override=0):
+        return Volume(self, data, brick_size_override)
+
+
+class Volume:
+    """A device-resident .csdb volume (fcs_load_csdb)."""
+
+    def __init__(self, ctx, data, brick_size_override=0):
+        self.ctx = ctx
+        buf = np.frombuffer(bytes(data), dtype=np.uint8)
+        h = c_vp()
+        check(lib.fcs_load_csdb(ctx.h, _ptr(buf, c_vp), buf.size, brick_size_override, ctypes.byref(h)), ctx.h)
+        self.h = h
+        self.info = self.get_info()
+        self.label_dtype = np.uint64 if self.info["label_bits"] == 64 else np.uint32
+
+    def close(self):
+        if self.h:
+            lib.fcs_volume_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_info(self):
+        i = VolumeInfo()
+        check(lib.fcs_volume_get_info(self.h, ctypes.byref(i)), self.ctx.h)
+        d = {f: getattr(i, f) for f, _ in VolumeInfo._fields_}
+        d["rans_freq"] = list(i.rans_freq)
+        d["rans_cum"] = list(i.rans_cum)
+        return d
+
+    def decode_rans(self, raise_on_error=True):
+        st = np.zeros(self.info["num_bricks"], np.uint32)
+        rc = lib.fcs_decode_rans(self.h, _ptr(st, c_u32p))
+        if rc < 0 and (raise_on_error or rc != -6):
+            check(rc, self.ctx.h)
+        return st
+
+    def cache_reserve(self, voxels):
+        check(lib.fcs_cache_reserve(self.h, int(voxels)), self.ctx.h)
+
+    def cache_clear(self):
+        check(lib.fcs_cache_clear(self.h), self.ctx.h)
+
+    def request_bricks(self, bricks, lods, raise_on_error=True):
+        b = np.ascontiguousarray(bricks, dtype=np.uint32)
+        l = np.ascontiguousarray(lods, dtype=np.uint8)
+        off = np.zeros(b.size, np.uint64)
+        st = np.zeros(b.size, np.uint32)
+        rc = lib.fcs_request_bricks(self.h, _ptr(b, c_u32p), _ptr(l, c_u8p), b.size, _ptr(off, c_u64p),
+                                    _ptr(st, c_u32p))
+        if rc < 0 and (raise_on_error or rc != -6):
+            check(rc, self.ctx.h)
+        return off, st
+
+    def decode_all(self, lod=255, pyramid=True):
+        check(lib.fcs_decode_all(self.h, int(lod), 1 if pyramid else 0), self.ctx.h)
+
+    def read_cache(self, offset, count):
+        out = np.zeros(int(count), self.label_dtype)
+        check(lib.fcs_read_cache(self.h, int(offset), int(count), _ptr(out, c_vp)), self.ctx.h)
+        return out
+
+    def read_brick(self, brick, lod):
+        out = np.zeros(1 << (3 * lod), self.label_dtype)
+        check(lib.fcs_read_brick(self.h, int(brick), int(lod), _ptr(out, c_vp)), self.ctx.h)
+        return out
+
+    def brick_lods(self, cam, w, h):
+        out = np.zeros(self.info["num_bricks"], np.uint8)
+        check(lib.fcs_brick_lods(self.h, ctypes.byref(cam.s), w, h, _ptr(out, c_u8p)), self.ctx.h)
+        return out
+
+    def orbit_camera(self, yaw, pitch, radius, fov=math.pi / 3):
+        cam = Camera((0, 0, 0), (0, 0, 1), (1, 0, 0), (0, 1, 0))
+        check(lib.fcs_camera_orbit(self.h, yaw, pitch, radius, fov, ctypes.byref(cam.s)), self.ctx.h)
+        return cam
+
+    def render(self, cam, w, h, brick_lod=None, want=("label", "voxel", "t_hit", "brick_visited")):
+        n = w * h
+        nb = self.info["num_bricks"]
+        label = np.zeros(n, self.label_dtype) if "label" in want else None
+        voxel = np.zeros(n, np.uint64) if "voxel" in want else None
+        t_hit = np.zeros(n, np.float32) if "t_hit" in want else None
+        vis = np.zeros(nb, np.uint32) if "brick_visited" in want else None
+        used = np.zeros(nb, np.uint8)
+        f = FrameStruct()
+        f.label = label.ctypes.data if label is not None else None
+        f.voxel = _ptr(voxel, c_u64p) if voxel is not None else None
+        f.t_hit = t_hit.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if t_hit is not None else None
+        f.brick_visited = _ptr(vis, c_u32p) if vis is not None else None
+        f.brick_lod_used = _ptr(used, c_u8p)
+        lp = None
+        if brick_lod is not None:
+            bl = np.ascontiguousarray(brick_lod, dtype=np.uint8)
+            lp = _ptr(bl, c_u8p)
+        check(lib.fcs_render(self.h, ctypes.byref(cam.s), lp, w, h, ctypes.byref(f)), self.ctx.h)
+        return Frame(label, voxel, t_hit, vis, used, f.bricks_decoded, f.decode_ms, f.march_ms)
+
+
+def load_dataset_from_array_buffer(ctx, array_buffer, brick_size_override=0):
+    """loadDatasetFromArrayBuffer (web/scripts/datasetLoader.js:39-145):
+    parse a .csdb byte buffer; raises FcsError("Invalid file format") on a bad
+    magic, like the reference."""
+    return Volume(ctx, array_buffer, brick_size_override)
+
+
+def decode_all_rans_bricks(volume):
+    """decodeAllRansBricks (web/scripts/decode.js:17-26), on the device: every
+    brick's rANS stream becomes a device-resident nibble stream. Returns the
+    per-brick status words."""
+    return volume.decode_rans()

@@ -1,0 +1,182 @@
+/* fcs.h — C ABI of the MI355X brick decoder and DDA ray marcher.
+ *
+ * The reference exposes this path as a JavaScript module API plus WGSL bind
+ * groups (web/app.js:9-12). Each entry point below replaces one of them:
+ *
+ *   fcs_load_csdb         <- loadDatasetFromArrayBuffer (web/scripts/datasetLoader.js:39-145)
+ *                            and loadDatasetFromFile (encoder/CSVEncoder/src/fileIO.cpp:101-183)
+ *   fcs_decode_rans       <- decodeAllRansBricks (web/scripts/decode.js:17-26),
+ *                            decodeRansStream (encoder/CSVEncoder/src/brickDecoding.cpp:153-188)
+ *   fcs_request_bricks    <- the decode dispatch of one frame: updateWorkQueue + runComputePass
+ *                            (web/scripts/cacheProcessing.js:303-322, web/scripts/gpuPasses.js:15-30)
+ *                            over the decodeBrick kernel (web/shaders/decode.wgsl:344-508)
+ *   fcs_decode_all        <- initializeCache(forceAll) (web/scripts/renderLoop.js:134-181)
+ *   fcs_read_brick        <- decodeBrick<b> (encoder/CSVEncoder/src/brickDecoding.hpp:174-242)
+ *   fcs_render            <- frame(): LOD update + decode + rayTrace pass
+ *                            (web/scripts/renderLoop.js:212-359, web/shaders/rayTrace.wgsl:565-906)
+ *   fcs_brick_lods        <- calculateAllBrickLODs / calculateBrickLOD (web/scripts/camera.js:103-201)
+ *   fcs_camera_orbit      <- OrbitController.apply + Camera.getUniformData
+ *                            (web/scripts/orbitController.js:100-123, web/scripts/camera.js:61-77)
+ *
+ * Conventions
+ *   - Every function returns >= 0 on success and a negative FCS_E_* code on
+ *     error; fcs_last_error() then holds a message. No exceptions cross.
+ *   - The caller owns every host buffer; the library copies inputs.
+ *   - Device memory is owned by the context / volume.
+ *   - One context = one HIP stream; a context is not thread-safe. Separate
+ *     contexts are independent (one per serving session).
+ *   - Labels are returned as uint32 (label_bits 32) or uint64 (label_bits 64).
+ *   - Per-request status words use the FCS_ST_* bits (stream overrun, palette
+ *     out of range, nibble-count mismatch, cache full, rANS invariant).
+ */
+#ifndef FCS_H
+#define FCS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCS_OK 0
+#define FCS_E_INVALID -1    /* bad argument */
+#define FCS_E_FORMAT -2     /* not a .csdb stream / truncated / inconsistent */
+#define FCS_E_HIP -3        /* HIP runtime error */
+#define FCS_E_NOMEM -4      /* host or device allocation failed */
+#define FCS_E_NODEVICE -5   /* no HIP device */
+#define FCS_E_CORRUPT -6    /* a brick stream failed decoding (see status words) */
+#define FCS_E_CACHE_FULL -7 /* brick cache pool exhausted */
+
+/* Per-brick / per-request status bits (OR-ed together). */
+#define FCS_ST_OK 0u
+#define FCS_ST_STREAM_OVERRUN 1u /* op stream ended before the requested level */
+#define FCS_ST_PALETTE_RANGE 2u  /* palette cursor outside [0, P) */
+#define FCS_ST_NSYM_MISMATCH 4u  /* nibble count disagrees with the op structure */
+#define FCS_ST_CACHE_FULL 8u     /* brick cache pool exhausted */
+#define FCS_ST_RANS_STATE 16u    /* rANS end-of-stream invariant violated */
+#define FCS_ST_BAD_OP 32u        /* op code 7 (undefined by the format) */
+#define FCS_ST_BAD_REQUEST 64u   /* brick index or LOD out of range */
+
+typedef struct fcs_ctx fcs_ctx;
+typedef struct fcs_volume fcs_volume;
+
+typedef struct {
+  uint32_t num_bricks;      /* = bricks_per_axis^3 */
+  uint32_t brick_size;      /* b */
+  uint32_t levels;          /* L = log2(b); LOD t in [0, L] */
+  uint32_t bricks_per_axis;
+  uint32_t grid_size;       /* bricks_per_axis * b */
+  uint32_t label_bits;      /* 32 or 64 (64 when any palette entry >= 2^32) */
+  uint32_t header_brick_size; /* value stored in the file header */
+  uint32_t rans_ready;      /* 1 once fcs_decode_rans has run */
+  uint64_t compressed_bytes;  /* sum of rANS stream bytes */
+  uint64_t total_symbols;     /* sum of nSymbols */
+  uint64_t palette_entries;   /* sum of palette sizes */
+  uint64_t cache_capacity_voxels; /* brick cache pool size */
+  uint32_t rans_freq[16];
+  uint32_t rans_cum[16];
+} fcs_volume_info;
+
+/* Camera: the 64-byte uniform of camera.js:61-77 (position, tanHalfFov,
+ * forward, right, up) plus an orthographic switch. ortho_half_height > 0
+ * selects an orthographic camera: ray origin = position + right*ndc_x*aspect*h
+ * + up*ndc_y*h, direction = forward. */
+typedef struct {
+  float position[3];
+  float tan_half_fov;
+  float forward[3];
+  float ortho_half_height;
+  float right[3];
+  float pad1;
+  float up[3];
+  float pad2;
+} fcs_camera;
+
+/* Per-frame outputs (host pointers, each nullable). */
+typedef struct {
+  void* label;             /* w*h labels (uint32 or uint64 per label_bits); 0 = no hit */
+  uint64_t* voxel;         /* w*h linear voxel index x + G*(y + G*z) of the hit
+                              voxel's LOD-cell min corner; UINT64_MAX = no hit */
+  float* t_hit;            /* w*h ray parameter at entry of the hit cell; -1 = no hit */
+  uint32_t* brick_visited; /* num_bricks flags: 1 = a ray entered this non-empty brick */
+  uint8_t* brick_lod_used; /* num_bricks LODs used this frame */
+  /* filled by the library */
+  uint32_t bricks_decoded; /* decode requests issued for this frame (cache misses) */
+  uint32_t pad;
+  float decode_ms;         /* device time of this frame's decode (hipEvent) */
+  float march_ms;          /* device time of this frame's march (hipEvent) */
+} fcs_frame;
+
+typedef struct {
+  float rans_ms;          /* last fcs_decode_rans */
+  float decode_ms;        /* last fcs_request_bricks / fcs_decode_all */
+  float march_ms;         /* last fcs_render march */
+  float pad;
+  uint64_t decoded_voxels;  /* labels written by the last decode */
+  uint64_t decode_bytes;    /* algorithmic bytes of the last decode */
+  uint64_t rans_symbols;    /* symbols decoded by the last fcs_decode_rans */
+  uint64_t rans_bytes;      /* compressed bytes consumed by the last fcs_decode_rans */
+} fcs_timing;
+
+int fcs_create(int hip_device, fcs_ctx** out);
+void fcs_destroy(fcs_ctx* ctx);
+const char* fcs_last_error(const fcs_ctx* ctx);
+int fcs_device_count(void);
+
+/* Parse a .csdb byte stream and upload it. brick_size_override != 0 replaces
+ * the header brick size (the reference encoder always writes 64). */
+int fcs_load_csdb(fcs_ctx* ctx, const uint8_t* bytes, size_t n, uint32_t brick_size_override,
+                  fcs_volume** out);
+void fcs_volume_free(fcs_volume* vol);
+int fcs_volume_get_info(const fcs_volume* vol, fcs_volume_info* out);
+
+/* Run the device rANS stage for every brick: nibble streams + per-level side
+ * table. out_status (nullable) receives num_bricks status words. Returns
+ * FCS_E_CORRUPT if any brick failed. */
+int fcs_decode_rans(fcs_volume* vol, uint32_t* out_status);
+
+/* Size the device brick cache pool (voxels). Drops resident bricks. */
+int fcs_cache_reserve(fcs_volume* vol, uint64_t voxels);
+/* Drop every resident brick (pool kept). */
+int fcs_cache_clear(fcs_volume* vol);
+
+/* Decode n (brick, LOD) requests into the brick cache (one batched launch).
+ * Bricks already resident at that LOD are not decoded again.
+ * out_cache_offset (nullable): voxel offset of each request's labels in the pool.
+ * out_status (nullable): per-request FCS_ST_* bits. */
+int fcs_request_bricks(fcs_volume* vol, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                       uint64_t* out_cache_offset, uint32_t* out_status);
+
+/* Decode every brick at LOD `lod` (lod > levels: every brick at its finest LOD).
+ * pyramid != 0 writes all levels 0..lod of each brick (coarse-first layout). */
+int fcs_decode_all(fcs_volume* vol, uint32_t lod, int pyramid);
+
+/* Copy the labels of `brick` at `lod` to the host (8^lod labels, Morton order),
+ * decoding it first if it is not resident. */
+int fcs_read_brick(fcs_volume* vol, uint32_t brick, uint32_t lod, void* host_labels);
+
+/* Copy the whole brick cache pool region [offset, offset+count) to the host. */
+int fcs_read_cache(fcs_volume* vol, uint64_t offset, uint64_t count, void* host_labels);
+
+/* Per-brick LOD by camera.js:103-137 (f64) for a w x h viewport. */
+int fcs_brick_lods(const fcs_volume* vol, const fcs_camera* cam, uint32_t w, uint32_t h,
+                   uint8_t* out_lods);
+
+/* Render one frame: brick_lod (nullable: LOD rule) selects each brick's LOD;
+ * non-resident bricks at that LOD are decoded first (on demand), then every
+ * pixel is marched. */
+int fcs_render(fcs_volume* vol, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w,
+               uint32_t h, fcs_frame* out);
+
+/* Orbit pose around the volume centre (orbitController.js:100-123) with the
+ * basis of camera.js:61-77, fov in radians. */
+int fcs_camera_orbit(const fcs_volume* vol, double yaw, double pitch, double radius, double fov,
+                     fcs_camera* out);
+
+int fcs_get_timing(const fcs_ctx* ctx, fcs_timing* out);
+int fcs_sync(fcs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,575 @@
+/* oracle.c — CPU restatement of the reference brick decoder and DDA marcher.
+ * TEST INFRASTRUCTURE ONLY (the checker): tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg. The product never links or calls this file.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math), so that
+ * FP32 ray setup rounds exactly like the device code.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* OR_MUTANT (tests only): build a deliberately wrong oracle to show that the
+ * golden checks catch plausible mistakes. 0 = correct. */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+
+#define ST_OVERRUN 1u
+#define ST_PALETTE 2u
+#define ST_NSYM 4u
+#define ST_RANS 16u
+#define ST_BADOP 32u
+
+/* ------------------------------------------------------------ Morton ---- */
+/* utils.cpp:4-42 */
+static uint32_t part1by2(uint32_t n) {
+  n &= 0x3ff;
+  n = (n | (n << 16)) & 0x30000ff;
+  n = (n | (n << 8)) & 0x300f00f;
+  n = (n | (n << 4)) & 0x30c30c3;
+  n = (n | (n << 2)) & 0x9249249;
+  return n;
+}
+static uint32_t compact1by2(uint32_t n) {
+  n &= 0x9249249;
+  n = (n ^ (n >> 2)) & 0x30c30c3;
+  n = (n ^ (n >> 4)) & 0x300f00f;
+  n = (n ^ (n >> 8)) & 0x30000ff;
+  n = (n ^ (n >> 16)) & 0x3ff;
+  return n;
+}
+static uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return part1by2(x) | (part1by2(y) << 1) | (part1by2(z) << 2);
+}
+
+/* --------------------------------------------------------- .csdb parse -- */
+/* fileIO.cpp:101-183 and datasetLoader.js:39-145 (sort by ID: line 143) */
+static uint32_t rd32(const uint8_t* p) {
+  return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+static int cmp_brick(const void* a, const void* b) {
+  const or_brick* x = (const or_brick*)a;
+  const or_brick* y = (const or_brick*)b;
+  return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+int or_parse_csdb(const uint8_t* bytes, size_t n, uint32_t brick_size_override, or_dataset* out) {
+  memset(out, 0, sizeof *out);
+  if (n < 16 + 8) return -2;
+  if (rd32(bytes) != 0x43534244u) return -2;
+  out->version = rd32(bytes + 4);
+  out->num_bricks = rd32(bytes + 8);
+  out->header_brick_size = rd32(bytes + 12);
+  out->brick_size = brick_size_override ? brick_size_override : out->header_brick_size;
+  size_t p = 16;
+  const uint32_t symbols = rd32(bytes + p);
+  out->total = rd32(bytes + p + 4);
+  p += 8;
+  if (symbols != 16 || p + 128 > n) return -2;
+  for (int i = 0; i < 16; ++i) out->freq[i] = rd32(bytes + p + 4 * i);
+  for (int i = 0; i < 16; ++i) out->cum[i] = rd32(bytes + p + 64 + 4 * i);
+  out->cum[16] = out->total;
+  p += 128;
+  out->bricks = (or_brick*)calloc(out->num_bricks ? out->num_bricks : 1, sizeof(or_brick));
+  /* first pass: count palette entries */
+  uint64_t pal_total = 0;
+  size_t q = p;
+  for (uint32_t i = 0; i < out->num_bricks; ++i) {
+    if (q + 8 > n) return -2;
+    const uint32_t P = rd32(bytes + q + 4);
+    q += 8;
+    if (q + 8ull * P + 4 > n) return -2;
+    q += 8ull * P;
+    const uint32_t E = rd32(bytes + q);
+    q += 4;
+    if (q + (uint64_t)E + 4 > n) return -2;
+    q += E + 4;
+    pal_total += P;
+  }
+  out->palette = (uint64_t*)malloc((pal_total ? pal_total : 1) * sizeof(uint64_t));
+  uint64_t po = 0;
+  for (uint32_t i = 0; i < out->num_bricks; ++i) {
+    or_brick* b = &out->bricks[i];
+    b->id = rd32(bytes + p);
+    b->P = rd32(bytes + p + 4);
+    p += 8;
+    memcpy(out->palette + po, bytes + p, 8ull * b->P);
+    b->pal_off = po;
+    po += b->P;
+    p += 8ull * b->P;
+    b->E = rd32(bytes + p);
+    p += 4;
+    b->str_off = p;
+    p += b->E;
+    b->nsym = rd32(bytes + p);
+    p += 4;
+  }
+  qsort(out->bricks, out->num_bricks, sizeof(or_brick), cmp_brick);
+  out->src = bytes;
+  return 0;
+}
+
+void or_free(or_dataset* d) {
+  free(d->bricks);
+  free(d->palette);
+  memset(d, 0, sizeof *d);
+}
+
+/* ---------------------------------------------------------------- rANS -- */
+/* findSymbol / ransDecodeSymbol / decodeRansStream, brickDecoding.cpp:126-188 */
+static uint32_t find_symbol(uint32_t x, const or_dataset* d) {
+  uint32_t s = 0;
+  while (s + 1 < 16 && x >= d->cum[s + 1]) ++s;
+  return s;
+}
+static uint32_t rans_step(uint32_t* state, const uint8_t* in, size_t* pos, const or_dataset* d) {
+  const uint32_t x = *state % d->total;
+  const uint32_t s = find_symbol(x, d);
+  *state = d->freq[s] * (*state / d->total) + (x - d->cum[s]);
+  while (*state < (1u << 23) && *pos > 0) *state = (*state << 8) | in[--*pos];
+  return s;
+}
+
+size_t or_rans_decode(const or_dataset* d, uint32_t brick, uint8_t* out, uint32_t* status) {
+  const or_brick* b = &d->bricks[brick];
+  const uint8_t* in = d->src + b->str_off;
+  *status = 0;
+  if (b->E < 4) { *status |= ST_RANS; return 0; }
+  size_t pos = b->E;
+  const uint32_t b0 = in[--pos], b1 = in[--pos], b2 = in[--pos], b3 = in[--pos];
+  uint32_t state = (b0 << 24) | (b1 << 16) | (b2 << 8) | b3;
+  size_t nout = 0;
+  uint32_t done = 0;
+  while (done < b->nsym) {
+    const uint32_t hi = rans_step(&state, in, &pos, d) & 0xF;
+    ++done;
+    uint32_t lo = 0;
+    if (done < b->nsym) { lo = rans_step(&state, in, &pos, d) & 0xF; ++done; }
+    out[nout++] = (uint8_t)((hi << 4) | lo);
+  }
+  /* the encoder coded an even symbol count (a 0 pad after an odd count) and
+   * started from state 2^23, so after the pad the state returns to 2^23 with
+   * every byte consumed */
+  if (b->nsym & 1) {
+    const uint32_t pad = rans_step(&state, in, &pos, d);
+    if (pad != 0) *status |= ST_RANS;
+  }
+  if (state != (1u << 23) || pos != 0) *status |= ST_RANS;
+  return nout;
+}
+
+/* --------------------------------------------------------- op decoder --- */
+/* decode.wgsl:239-508 with brickDecoding.cpp:7-124 semantics; explicit
+ * `filled` flags replace the in-band EMPTY sentinel (decode.wgsl:46). */
+static uint32_t node_to_out(uint32_t m, uint32_t l, uint32_t t) {
+  /* coarseNodeToOutputIndex (brickDecoding.cpp:7-18) */
+  const uint32_t s = 1u << (t - l);
+  return morton3(compact1by2(m) * s, compact1by2(m >> 1) * s, compact1by2(m >> 2) * s);
+}
+
+typedef struct {
+  const uint8_t* nib;
+  size_t nbytes, pos;
+  int high;
+} reader;
+
+static int read_nibble(reader* r, uint32_t* v) {
+  /* readNibble (brickDecoding.cpp:20-41): low nibble of byte 0 first */
+  if (r->pos >= r->nbytes) return 0;
+  const uint8_t byte = r->nib[r->pos];
+  if (r->high) { *v = byte >> 4; r->high = 0; } else { *v = byte & 0xF; r->high = 1; r->pos++; }
+  return 1;
+}
+
+int or_decode_brick(const uint8_t* nib, size_t nbytes, const uint64_t* pal, uint32_t P, uint32_t t,
+                    uint64_t* out, uint32_t* status) {
+  *status = 0;
+  const uint64_t n = 1ull << (3 * t);
+  uint8_t* filled = (uint8_t*)calloc(n, 1);
+  if (!filled) return -4;
+  const int root_stop = (OR_MUTANT == 3) ? 0 : (nbytes > 0 ? ((nib[0] >> 4) & 1) : 0);
+  if (P == 0) { *status |= ST_PALETTE; free(filled); return 0; }
+  out[0] = pal[0];
+  filled[0] = 1;
+  if (root_stop) {
+    /* set root stop bit: constant brick, every LOD = palette[0] */
+    for (uint64_t i = 0; i < n; ++i) out[i] = pal[0];
+    free(filled);
+    return 0;
+  }
+  reader r = {nib, nbytes, 0, 0};
+  int64_t ip = 0;
+  for (uint32_t l = 0; l < t; ++l) {
+    const uint32_t count = 1u << (3 * l);
+    for (uint32_t m = 0; m < count; ++m) {
+      const uint32_t i = node_to_out(m, l, t);
+      const uint32_t last = node_to_out(8 * m + 7, l + 1, t);
+      if (filled[last]) continue; /* alreadyFilled (decode.wgsl:433) */
+      const uint64_t parent = out[i];
+      for (uint32_t c = 0; c < 8; ++c) {
+        const uint32_t child = 8 * m + c;
+        const uint32_t j = node_to_out(child, l + 1, t);
+        uint32_t nibv, delta = 0;
+        if (!read_nibble(&r, &nibv)) { *status |= ST_OVERRUN; goto done; }
+        const uint32_t op = nibv >> 1, stop = nibv & 1;
+        if (op == 6 && !read_nibble(&r, &delta)) { *status |= ST_OVERRUN; goto done; }
+        uint64_t val = 0;
+        switch (op) {
+          case 0: val = parent; break;
+          case 1: case 2: case 3: {
+            /* getNeighborValue (brickDecoding.cpp:77-124) */
+            const int32_t cx = (int32_t)compact1by2(child), cy = (int32_t)compact1by2(child >> 1),
+                          cz = (int32_t)compact1by2(child >> 2);
+            int32_t nx = cx, ny = cy, nz = cz;
+            if (op == 1) nx = (cx & 1) == 0 ? cx - 1 : cx + 1;
+            if (op == 2) ny = (cy & 1) == 0 ? cy - 1 : cy + 1;
+            if (op == 3) nz = (cz & 1) == 0 ? cz - 1 : cz + 1;
+            const int32_t maxc = (int32_t)((1u << (l + 1)) - 1);
+            if (nx < 0 || ny < 0 || nz < 0 || nx > maxc || ny > maxc || nz > maxc) {
+              val = out[node_to_out(morton3(cx >> 1, cy >> 1, cz >> 1), l, t)];
+            } else {
+              const uint32_t nm = morton3(nx, ny, nz);
+              if (nm < child || OR_MUTANT == 1) val = out[node_to_out(nm, l + 1, t)];
+              else val = out[node_to_out(morton3(nx >> 1, ny >> 1, nz >> 1), l, t)];
+            }
+            break;
+          }
+          case 4:
+            if (OR_MUTANT != 5) ++ip;
+            if (ip >= P) { *status |= ST_PALETTE; val = pal[P - 1]; } else val = pal[ip];
+            break;
+          case 5:
+            if (ip >= P) { *status |= ST_PALETTE; val = pal[P - 1]; } else val = pal[ip];
+            break;
+          case 6: {
+            const int64_t k = ip - (int64_t)delta - (OR_MUTANT == 2 ? 0 : 1);
+            if (k < 0 || k >= P) { *status |= ST_PALETTE; val = pal[P - 1]; } else val = pal[k];
+            break;
+          }
+          default:
+            *status |= ST_BADOP;
+            val = 0;
+        }
+        out[j] = val;
+        filled[j] = 1;
+        if (stop) {
+          /* fillSubBlock (brickDecoding.cpp:61-75) */
+          const uint64_t bs = 1ull << (3 * (t - l - 1));
+          for (uint64_t k = 0; k < bs; ++k) { out[j + k] = val; filled[j + k] = 1; }
+        }
+      }
+    }
+  }
+done:
+  free(filled);
+  return 0;
+}
+
+int or_decode_dataset_brick(const or_dataset* d, uint32_t brick, uint32_t t, uint64_t* out,
+                            uint32_t* status) {
+  const or_brick* b = &d->bricks[brick];
+  uint8_t* nib = (uint8_t*)malloc((b->nsym + 2) / 2 + 1);
+  uint32_t st1 = 0, st2 = 0;
+  const size_t nb = or_rans_decode(d, brick, nib, &st1);
+  const int rc = or_decode_brick(nib, nb, d->palette + b->pal_off, b->P, t, out, &st2);
+  free(nib);
+  *status = st1 | st2;
+  return rc;
+}
+
+/* ------------------------------------------------------------- LOD rule - */
+/* calculateBrickLOD (camera.js:103-137) */
+uint32_t or_brick_lod(double cx, double cy, double cz, double brick_size, double px, double py,
+                      double pz, double fov, double screen_width, uint32_t max_lod) {
+  const double dx = cx - px, dy = cy - py, dz = cz - pz;
+  double d = sqrt(dx * dx + dy * dy + dz * dz);
+  if (d < 0.1) d = 0.1;
+  const double focal = (screen_width / 2) / tan(fov / 2);
+  uint32_t best = 0;
+  for (uint32_t lod = 0; lod <= max_lod; ++lod) {
+    const double voxel = brick_size / pow(2.0, (double)lod);
+    const double pixels = (focal * voxel) / d;
+    if (pixels >= 1.0) best = lod; else break;
+  }
+  return best;
+}
+
+/* ------------------------------------------------------------------ DDA - */
+/* rayTrace.wgsl:457-563 and the traversal of 565-892, FP32 throughout,
+ * with the zero-direction fix: a zero component has tMax = tDelta = +inf
+ * and no slab constraint when the origin lies within that slab. */
+typedef struct { float o[3], d[3], inv[3]; int zero[3]; } ray_t;
+
+static float fmaxf_(float a, float b) { return a > b ? a : b; }
+static float fminf_(float a, float b) { return a < b ? a : b; }
+
+static int intersect_volume(const ray_t* r, float G, float* tenter, float* texit) {
+  float tmin = -INFINITY, tmax = INFINITY;
+  for (int a = 0; a < 3; ++a) {
+    if (r->zero[a]) {
+      if (r->o[a] < 0.0f || r->o[a] > G) return 0;
+      continue;
+    }
+    const float t0 = (0.0f - r->o[a]) * r->inv[a];
+    const float t1 = (G - r->o[a]) * r->inv[a];
+    tmin = fmaxf_(tmin, fminf_(t0, t1));
+    tmax = fminf_(tmax, fmaxf_(t0, t1));
+  }
+  if (tmax < 0.0f || tmin > tmax) return 0;
+  *tenter = fmaxf_(tmin, 0.0f);
+  *texit = tmax;
+  return 1;
+}
+
+static float adv(int v[3], float tmax[3], const float tdelta[3], const int step[3]) {
+  /* advanceDDA / advanceBrickDDA: ties prefer z, then y (strict <) */
+  float t;
+  if (tmax[0] < tmax[1] && tmax[0] < tmax[2]) { t = tmax[0]; tmax[0] += tdelta[0]; v[0] += step[0]; }
+  else if (tmax[1] < tmax[2]) { t = tmax[1]; tmax[1] += tdelta[1]; v[1] += step[1]; }
+  else { t = tmax[2]; tmax[2] += tdelta[2]; v[2] += step[2]; }
+  return t;
+}
+
+static float bias(float t) { return fmaxf_(t * 1e-6f, 1e-5f); }
+
+int or_render(uint32_t B, uint32_t b, const uint8_t* brick_empty, const uint8_t* brick_lod,
+              const uint64_t* const* brick_labels, const uint64_t* palette0, const or_camera* cam,
+              uint32_t w, uint32_t h, uint64_t* out_label, uint64_t* out_voxel, float* out_t,
+              uint32_t* visited) {
+  const float Wf = (float)w, Hf = (float)h;
+  const float aspect = (float)((double)w / (double)h);
+  const float G = (float)(B * b);
+  const uint64_t Gi = (uint64_t)B * b;
+  for (uint32_t py = 0; py < h; ++py) {
+    for (uint32_t px = 0; px < w; ++px) {
+      const size_t pix = (size_t)py * w + px;
+      uint64_t hit_label = 0, hit_vox = UINT64_MAX;
+      float hit_t = -1.0f;
+      /* rayTrace.wgsl:597-619 */
+      float ndc_x = ((float)px + 0.5f) / Wf;
+      ndc_x = ndc_x * 2.0f;
+      ndc_x = ndc_x - 1.0f;
+      float ndc_y = ((float)py + 0.5f) / Hf;
+      ndc_y = ndc_y * 2.0f;
+      ndc_y = 1.0f - ndc_y;
+      ray_t r;
+      if (cam->ortho_half_height > 0.0f) {
+        for (int a = 0; a < 3; ++a) {
+          float ox = cam->right[a] * ndc_x;
+          ox = ox * aspect;
+          ox = ox * cam->ortho_half_height;
+          float oy = cam->up[a] * ndc_y;
+          oy = oy * cam->ortho_half_height;
+          r.o[a] = (cam->position[a] + ox) + oy;
+          r.d[a] = cam->forward[a];
+        }
+      } else {
+        float v[3];
+        for (int a = 0; a < 3; ++a) {
+          float rt = cam->right[a] * ndc_x;
+          rt = rt * aspect;
+          rt = rt * cam->tan_half_fov;
+          float ut = cam->up[a] * ndc_y;
+          ut = ut * cam->tan_half_fov;
+          v[a] = (cam->forward[a] + rt) + ut;
+          r.o[a] = cam->position[a];
+        }
+        float dd = v[0] * v[0];
+        dd = dd + v[1] * v[1];
+        dd = dd + v[2] * v[2];
+        const float len = sqrtf(dd);
+        for (int a = 0; a < 3; ++a) r.d[a] = v[a] / len;
+      }
+      for (int a = 0; a < 3; ++a) {
+        r.zero[a] = r.d[a] == 0.0f;
+        const float s = r.d[a] > 0.0f ? 1.0f : (r.d[a] < 0.0f ? -1.0f : 0.0f);
+        r.inv[a] = r.zero[a] ? INFINITY : s / fmaxf_(fabsf(r.d[a]), 1e-6f);
+      }
+      float tEnter, tExit;
+      if (!intersect_volume(&r, G, &tEnter, &tExit)) goto store;
+      {
+        /* initBrickDDA (rayTrace.wgsl:519-545) */
+        int step[3], brick[3];
+        float btmax[3], btdelta[3];
+        {
+          const float ts = tEnter + bias(tEnter);
+          for (int a = 0; a < 3; ++a) {
+            const float pos = r.o[a] + r.d[a] * ts;
+            brick[a] = (int)floorf(pos) / (int)b;
+            step[a] = r.d[a] > 0.0f ? 1 : (r.d[a] < 0.0f ? -1 : 0);
+            const float bf = (float)brick[a] * (float)b;
+            const float nb = r.d[a] > 0.0f ? bf + (float)b : bf;
+            btmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+            btdelta[a] = r.zero[a] ? INFINITY : fabsf((float)b * r.inv[a]);
+          }
+        }
+        int be[3], bx[3];
+        uint32_t maxb = 6;
+        for (int a = 0; a < 3; ++a) {
+          be[a] = (int)(r.o[a] + r.d[a] * tEnter) / (int)b;
+          bx[a] = (int)(r.o[a] + r.d[a] * tExit) / (int)b;
+          maxb += (uint32_t)abs(bx[a] - be[a]);
+        }
+        float brickEntryT = tEnter;
+        for (uint32_t it = 0; it < maxb; ++it) {
+          if (brick[0] < 0 || brick[1] < 0 || brick[2] < 0 || brick[0] >= (int)B || brick[1] >= (int)B ||
+              brick[2] >= (int)B)
+            break;
+          const uint32_t bi = morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
+          if (!brick_empty[bi]) {
+            if (visited) visited[bi] = 1;
+            const int bmin[3] = {brick[0] * (int)b, brick[1] * (int)b, brick[2] * (int)b};
+            const float tEnterBrick = fmaxf_(brickEntryT, tEnter);
+            float tExitBrick = fminf_(fminf_(btmax[0], btmax[1]), btmax[2]);
+            tExitBrick = fmaxf_(tEnterBrick, tExitBrick - bias(tExitBrick));
+            /* initDDA (rayTrace.wgsl:473-499) */
+            int vox[3], vstep[3];
+            float vtmax[3], vtdelta[3];
+            const float ts = tEnterBrick + bias(tEnterBrick);
+            for (int a = 0; a < 3; ++a) {
+              const float pos = r.o[a] + r.d[a] * ts;
+              vox[a] = (int)floorf(pos);
+              vstep[a] = step[a];
+              const float nb = r.d[a] > 0.0f ? (float)(vox[a] + 1) : (float)vox[a];
+              vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+              vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]);
+            }
+            const uint32_t lod = brick_lod[bi];
+            const int lodSize = 1 << lod;
+            const int stride = (int)b / lodSize;
+            for (int a = 0; a < 3; ++a) vox[a] -= bmin[a];
+            if (stride > 1 && OR_MUTANT != 4) {
+              for (int a = 0; a < 3; ++a) {
+                int lv = (vox[a] / stride) * stride;
+                lv = lv < 0 ? 0 : (lv > (int)b - 1 ? (int)b - 1 : lv);
+                vox[a] = lv;
+                /* the reference keeps a unit voxel step here (rayTrace.wgsl:721-744)
+                 * while tDelta covers `stride` voxels, so coarse cells are sampled
+                 * out of place; the cell walk steps by the stride */
+                vstep[a] = step[a] * stride;
+                vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]) * (float)stride;
+                const float lw = (float)bmin[a] + (float)vox[a];
+                const float nb = r.d[a] > 0.0f ? lw + (float)stride : lw;
+                vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+              }
+            } else {
+              for (int a = 0; a < 3; ++a) vox[a] = vox[a] < 0 ? 0 : (vox[a] > (int)b - 1 ? (int)b - 1 : vox[a]);
+            }
+            uint32_t maxs = 128;
+            for (int a = 0; a < 3; ++a) {
+              const float ve = r.o[a] + r.d[a] * tEnterBrick;
+              const float vx = r.o[a] + r.d[a] * tExitBrick;
+              const float le = (ve - (float)bmin[a]) / (float)stride;
+              const float lx = (vx - (float)bmin[a]) / (float)stride;
+              maxs += (uint32_t)abs((int)ceilf(lx) - (int)floorf(le));
+            }
+            float tcur = tEnterBrick;
+            const float lodScale = (float)lodSize / (float)b;
+            for (uint32_t s = 0; s < maxs; ++s) {
+              const int inb = vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < (int)b && vox[1] < (int)b &&
+                              vox[2] < (int)b;
+              if (inb) {
+                /* getVoxelColor (rayTrace.wgsl:160-204); LOD 0 = palette[0] */
+                uint64_t lab;
+                if (lod == 0) {
+                  lab = palette0[bi];
+                } else {
+                  uint32_t cc[3];
+                  for (int a = 0; a < 3; ++a) {
+                    float sc = (float)vox[a] * lodScale;
+                    sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
+                    cc[a] = (uint32_t)sc;
+                  }
+                  lab = brick_labels[bi][morton3(cc[0], cc[1], cc[2])];
+                }
+                if (lab != 0) {
+                  hit_label = lab;
+                  hit_vox = (uint64_t)(bmin[0] + vox[0]) +
+                            Gi * ((uint64_t)(bmin[1] + vox[1]) + Gi * (uint64_t)(bmin[2] + vox[2]));
+                  hit_t = tcur;
+                  goto store;
+                }
+              }
+              const float t = adv(vox, vtmax, vtdelta, vstep);
+              tcur = t;
+              if (t > tExitBrick) break;
+            }
+          }
+          const float t = adv(brick, btmax, btdelta, step);
+          brickEntryT = t + bias(t);
+          if (t > tExit) break;
+        }
+      }
+    store:
+      if (out_label) out_label[pix] = hit_label;
+      if (out_voxel) out_voxel[pix] = hit_vox;
+      if (out_t) out_t[pix] = hit_t;
+    }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------- threaded full decode ---- */
+typedef struct {
+  const or_dataset* d;
+  uint64_t* out;
+  uint32_t next;
+  pthread_mutex_t mu;
+  uint32_t status;
+  uint32_t L;
+} pool_t;
+
+static void* pool_worker(void* arg) {
+  pool_t* P = (pool_t*)arg;
+  const uint64_t per = ((1ull << (3 * (P->L + 1))) - 1) / 7;
+  uint8_t* nib = NULL;
+  size_t cap = 0;
+  uint32_t st_acc = 0;
+  for (;;) {
+    pthread_mutex_lock(&P->mu);
+    const uint32_t i = P->next++;
+    pthread_mutex_unlock(&P->mu);
+    if (i >= P->d->num_bricks) break;
+    const or_brick* b = &P->d->bricks[i];
+    const size_t need = (b->nsym + 2) / 2 + 1;
+    if (need > cap) { free(nib); nib = (uint8_t*)malloc(need); cap = need; }
+    uint32_t st = 0;
+    const size_t nb = or_rans_decode(P->d, i, nib, &st);
+    st_acc |= st;
+    for (uint32_t t = 0; t <= P->L; ++t) {
+      uint32_t s2 = 0;
+      or_decode_brick(nib, nb, P->d->palette + b->pal_off, b->P, t,
+                      P->out + per * i + ((1ull << (3 * t)) - 1) / 7, &s2);
+      st_acc |= s2;
+    }
+  }
+  free(nib);
+  pthread_mutex_lock(&P->mu);
+  P->status |= st_acc;
+  pthread_mutex_unlock(&P->mu);
+  return NULL;
+}
+
+uint64_t or_decode_all_pyramid(const or_dataset* d, uint64_t* out, int threads, uint32_t* status_or) {
+  uint32_t L = 0;
+  while ((1u << L) < d->brick_size) ++L;
+  pool_t P;
+  P.d = d;
+  P.out = out;
+  P.next = 0;
+  P.status = 0;
+  P.L = L;
+  pthread_mutex_init(&P.mu, NULL);
+  if (threads < 1) threads = 1;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, pool_worker, &P);
+  for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  pthread_mutex_destroy(&P.mu);
+  if (status_or) *status_or = P.status;
+  return (uint64_t)d->num_bricks * (((1ull << (3 * (L + 1))) - 1) / 7);
+}

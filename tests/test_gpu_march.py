@@ -1,0 +1,108 @@
+"""Device DDA marcher parity through the C ABI: per pixel first-hit label,
+voxel index and hit t against the oracle's FP32 DDA, bit-exact (mismatching
+pixels are counted and reported; the bar is zero)."""
+import math
+
+import numpy as np
+import pytest
+
+import fcs
+from conftest import golden_bytes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_frame(ds, pyr, lods, cam, w, h):
+    B = round(ds.num_bricks ** (1 / 3))
+    labels, empty, p0 = [], [], []
+    for i in range(ds.num_bricks):
+        t = int(lods[i])
+        labels.append(pyr[i][O.pyramid_offset(t):O.pyramid_offset(t + 1)])
+        pal = ds.palette(i)
+        empty.append(int(np.all(pal == 0)))
+        p0.append(pal[0])
+    return O.render(B, ds.brick_size, empty, lods, labels, p0, cam, w, h)
+
+
+def compare(frame, ref, tag):
+    lab, vox, tt, vis = ref
+    bad = (frame.label.astype(np.uint64) != lab) | (frame.voxel != vox) | (frame.t_hit.view(np.uint32) != tt.view(np.uint32))
+    nbad = int(bad.sum())
+    print(f"[gpu] {tag}: {frame.label.size} px, {nbad} mismatching, hits {int((lab != 0).sum())}", flush=True)
+    assert nbad == 0
+    assert np.array_equal(frame.brick_visited, vis)
+
+
+@pytest.fixture(scope="module")
+def cfg1(gpu_ctx):
+    data = golden_bytes("cfg1_b32")
+    vol = gpu_ctx.load(data, 32)
+    vol.decode_all(pyramid=True)
+    ds = O.Dataset(data, 32)
+    pyr, _ = ds.decode_all_pyramid()
+    return vol, ds, pyr
+
+
+@pytest.mark.parametrize("lod", [5, 3, 1])
+def test_ortho_frame_config1(cfg1, lod):
+    vol, ds, pyr = cfg1
+    cam = fcs.Camera.ortho_plus_z(128)
+    lods = np.full(64, lod, np.uint8)
+    f = vol.render(cam, 512, 512, brick_lod=lods)
+    assert f.bricks_decoded == 0  # pyramid resident
+    compare(f, oracle_frame(ds, pyr, lods, cam.as_dict(), 512, 512), f"ortho cfg1 lod {lod}")
+
+
+def test_perspective_mixed_lods_config1(cfg1):
+    vol, ds, pyr = cfg1
+    rng = np.random.default_rng(11)
+    lods = rng.integers(0, 6, 64).astype(np.uint8)
+    for k in range(3):
+        cam = vol.orbit_camera(0.3 + k, 0.2 * (k - 1), 1.5 * math.sqrt(3) * 128)
+        f = vol.render(cam, 320, 240, brick_lod=lods)
+        compare(f, oracle_frame(ds, pyr, lods, cam.as_dict(), 320, 240), f"persp cfg1 pose {k}")
+
+
+@pytest.fixture(scope="module")
+def cfg2(gpu_ctx):
+    print("[gpu] generating config 2 for the marcher", flush=True)
+    g = fcs.Generator(2)
+    data = fcs.encode_volume(g.volume(), 64)
+    vol = gpu_ctx.load(data)
+    vol.decode_all(pyramid=True)
+    ds = O.Dataset(data)
+    pyr, _ = ds.decode_all_pyramid()
+    return vol, ds, pyr
+
+
+def test_lod_rule_matches_oracle(cfg2):
+    vol, ds, _ = cfg2
+    cam = vol.orbit_camera(math.pi / 4, 0.15 * math.pi, 1.5 * math.sqrt(3) * 512)
+    lods = vol.brick_lods(cam, 1920, 1080)
+    fov = 2 * math.atan(cam.s.tan_half_fov)
+    for i in range(0, 512, 7):
+        bx = sum(((i >> (3 * k)) & 1) << k for k in range(3))
+        by = sum(((i >> (3 * k + 1)) & 1) << k for k in range(3))
+        bz = sum(((i >> (3 * k + 2)) & 1) << k for k in range(3))
+        c = (bx * 64 + 32.0, by * 64 + 32.0, bz * 64 + 32.0)
+        want = O.brick_lod(c, 64.0, tuple(float(p) for p in cam.s.position), fov, 1080.0, 6)
+        assert lods[i] == want
+
+
+def test_orbit_poses_config2(cfg2):
+    vol, ds, pyr = cfg2
+    r = 1.5 * math.sqrt(3) * 512
+    for k in range(8):
+        cam = vol.orbit_camera(math.pi / 4 + k * math.pi / 4, 0.15 * math.pi, r)
+        lods = vol.brick_lods(cam, 480, 270)
+        f = vol.render(cam, 480, 270)
+        assert np.array_equal(f.brick_lod_used, lods)
+        compare(f, oracle_frame(ds, pyr, lods, cam.as_dict(), 480, 270), f"orbit cfg2 pose {k}")
+
+
+def test_full_hd_frame_config2(cfg2):
+    vol, ds, pyr = cfg2
+    cam = vol.orbit_camera(math.pi / 4, 0.15 * math.pi, 1.5 * math.sqrt(3) * 512)
+    f = vol.render(cam, 1920, 1080)
+    compare(f, oracle_frame(ds, pyr, f.brick_lod_used, cam.as_dict(), 1920, 1080), "1080p cfg2")
