@@ -78,6 +78,8 @@ def main():
             marches.append(f.march_ms)
         return tr["rans_ms"], td["decode_ms"], marches, td["decoded_voxels"], td["decode_bytes"]
 
+    first = vol.decode_rans()  # autotunes the rANS mapping on this volume
+    tune = ctx.timing()
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -135,6 +137,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None},
         "kernels": kernels,
+        "rans_autotune": {"mapping": {1: "lane per brick", 2: "wave per brick"}.get(tune["rans_mapping"]),
+                          "lane_ms": tune["tune_lane_ms"], "wave_ms": tune["tune_wave_ms"]},
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O

@@ -32,7 +32,8 @@ def build(verbose=True):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for src, kind in (("fcs_kernels.hip", "hip"), ("fcs_host.cpp", "host"), ("fcs_producer.cpp", "host")):
+    for src, kind in (("fcs_rans.hip", "hip"), ("fcs_ops.hip", "hip"), ("fcs_march.hip", "hip"),
+                      ("fcs_host.cpp", "host"), ("fcs_producer.cpp", "host")):
         obj = os.path.join(objdir, src + ".o")
         if kind == "hip":
             _run(["hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
@@ -42,7 +43,21 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "--offload-arch=gfx950", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    build_node_addon()
     return LIB
+
+
+def build_node_addon():
+    """N-API addon (node/fcs.node) linking libfcs.so; skipped without node headers."""
+    inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(inc, "node_api.h")):
+        print("node headers not found; N-API addon not built", flush=True)
+        return None
+    out = os.path.join(HERE, "node", "fcs.node")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I", inc, "-I", os.path.join(ROOT, "include"),
+          "-DNODE_GYP_MODULE_NAME=fcs", os.path.join(HERE, "node", "fcs_napi.c"), "-o", out, "-L", LIBDIR,
+          "-lfcs", "-Wl,-rpath,$ORIGIN/../lib"])
+    return out
 
 
 if __name__ == "__main__":

@@ -12,19 +12,10 @@ struct DevBrick {
   uint32_t P;        // palette size
 };
 
-// Per-brick, per-op-level side table written by the rANS kernel:
-// entry l (0 <= l <= L) = nibble index where level l's ops start and the
-// palette cursor before them. Entry L = end of the last level.
-struct LevelEntry {
-  uint32_t nib_start;
-  uint32_t ip_base;
-};
-
-// Per-brick rANS results: status bits and the number of complete op levels
-// (a request at LOD t needs t <= levels_ok).
+// Per-brick rANS result: FCS_ST_* status bits.
 struct RansResult {
   uint32_t status;
-  uint32_t levels_ok;
+  uint32_t pad;
 };
 
 // One decode request. 16 bytes.

@@ -22,6 +22,7 @@ struct fcs_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
   fcs_timing timing{};
+  int rans_mapping = FCS_RANS_MAPPING_AUTO;
 };
 
 namespace {
@@ -40,7 +41,6 @@ struct fcs_volume {
   std::vector<DevBrick> hb;
   std::vector<uint8_t> empty;     // all palette entries are 0
   std::vector<uint64_t> pal0;     // palette[0] per brick
-  std::vector<uint8_t> levels_ok; // from the rANS stage
   std::vector<uint32_t> rans_status;
   // device
   uint8_t* d_stream = nullptr;
@@ -48,10 +48,10 @@ struct fcs_volume {
   void* d_pal = nullptr;
   void* d_pal0 = nullptr;
   uint32_t* d_nib = nullptr;
-  LevelEntry* d_lvl = nullptr;
   RansResult* d_rres = nullptr;
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
+  int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
   // brick cache pool (voxels of label_bits/8 bytes)
   void* d_cache = nullptr;
   uint64_t cap = 0, bump = 0;
@@ -119,12 +119,26 @@ bool cache_alloc(fcs_volume* v, uint64_t voxels, uint64_t* off) {
   return true;
 }
 
-void cache_release(fcs_volume* v, uint32_t brick) {
+// Releases are deferred to the end of a batch: a region dropped while the
+// batch is being built may still be the target of an earlier request of the
+// same launch, so it must not be handed to another request of that launch.
+void cache_release(fcs_volume* v, uint32_t brick, std::vector<Alloc>* deferred) {
   if (v->res_lod[brick] < 0) return;
   const Alloc a = v->res_alloc[brick];
-  const int cls = size_class(a.size);
-  if (cls >= 0) v->free_lists[cls].push_back(a.off);
+  if (deferred) {
+    deferred->push_back(a);
+  } else {
+    const int cls = size_class(a.size);
+    if (cls >= 0) v->free_lists[cls].push_back(a.off);
+  }
   v->res_lod[brick] = -1;
+}
+
+void cache_free_deferred(fcs_volume* v, const std::vector<Alloc>& deferred) {
+  for (const Alloc& a : deferred) {
+    const int cls = size_class(a.size);
+    if (cls >= 0) v->free_lists[cls].push_back(a.off);
+  }
 }
 
 int ensure_req(fcs_volume* v, size_t n) {
@@ -159,11 +173,11 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
   const uint32_t grid = (uint32_t)std::min<size_t>(reqs.size(), (size_t)c->num_cu * 2);
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
   if (v->label_bits == 32) {
-    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, v->d_lvl, v->d_rres, (const uint32_t*)v->d_pal, v->d_req,
+    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, v->d_req,
                         (uint32_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
     HIP_TRY(c, fcs_launch_ops<uint32_t>(a, grid, c->stream));
   } else {
-    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, v->d_lvl, v->d_rres, (const uint64_t*)v->d_pal, v->d_req,
+    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, v->d_req,
                         (uint64_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
     HIP_TRY(c, fcs_launch_ops<uint64_t>(a, grid, c->stream));
   }
@@ -243,6 +257,16 @@ int fcs_sync(fcs_ctx* c) {
   return 0;
 }
 
+int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
+  if (!c) return FCS_E_INVALID;
+  if (option == FCS_OPT_RANS_MAPPING) {
+    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE) return fail(c, FCS_E_INVALID, "bad mapping");
+    c->rans_mapping = (int)value;
+    return 0;
+  }
+  return fail(c, FCS_E_INVALID, "unknown option");
+}
+
 int fcs_get_timing(const fcs_ctx* c, fcs_timing* out) {
   if (!c || !out) return FCS_E_INVALID;
   *out = c->timing;
@@ -252,8 +276,9 @@ int fcs_get_timing(const fcs_ctx* c, fcs_timing* out) {
 int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_volume** out) {
   if (!c || !bytes || !out) return fail(c, FCS_E_INVALID, "null argument");
   *out = nullptr;
+  // magic first, with the reference loaders' message (fileIO.cpp:118-119, datasetLoader.js:76-77)
+  if (n < 4 || rd32(bytes) != FCS_MAGIC) return fail(c, FCS_E_FORMAT, "Invalid file format");
   if (n < 16 + 8 + 128) return fail(c, FCS_E_FORMAT, "stream too short for a .csdb header");
-  if (rd32(bytes) != FCS_MAGIC) return fail(c, FCS_E_FORMAT, "Invalid file format");
   HIP_TRY(c, hipSetDevice(c->device));
   auto v = new fcs_volume();
   v->ctx = c;
@@ -360,7 +385,6 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
   }
   if (rc) { fcs_volume_free(v); return rc; }
   if (hipMalloc(&v->d_nib, (nib_total + 8) * 4) != hipSuccess ||
-      hipMalloc(&v->d_lvl, (size_t)v->nb * (v->L + 1) * sizeof(LevelEntry)) != hipSuccess ||
       hipMalloc(&v->d_rres, (size_t)v->nb * sizeof(RansResult)) != hipSuccess ||
       hipMalloc(&v->d_counter, 64) != hipSuccess || hipMalloc(&v->d_meta, (size_t)v->nb * 4) != hipSuccess ||
       hipMalloc(&v->d_offs, (size_t)v->nb * 8) != hipSuccess) {
@@ -372,7 +396,6 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
   v->res_pyr.assign(v->nb, 0);
   v->res_alloc.assign(v->nb, Alloc{});
   v->free_lists.assign(16, {});
-  v->levels_ok.assign(v->nb, 0);
   v->rans_status.assign(v->nb, 0);
   *out = v;
   return 0;
@@ -381,7 +404,7 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
 void fcs_volume_free(fcs_volume* v) {
   if (!v) return;
   if (v->ctx) (void)hipSetDevice(v->ctx->device);
-  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_lvl, v->d_rres, v->d_counter,
+  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter,
                   v->d_cache, v->d_req, v->d_req_status, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel,
                   v->d_ft, v->d_visited, v->d_misses};
   for (void* q : ptrs)
@@ -422,23 +445,45 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   memcpy(a.freq, v->freq, sizeof a.freq);
   memcpy(a.cum, v->cum, sizeof a.cum);
   a.nib = v->d_nib;
-  a.lvl = v->d_lvl;
   a.res = v->d_rres;
-  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  HIP_TRY(c, fcs_launch_rans(a, c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  auto timed = [&](int mode, float* ms) -> int {
+    HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(c, fcs_launch_rans(a, mode, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev[3]));
+    HIP_TRY(c, hipEventElapsedTime(ms, c->ev[2], c->ev[3]));
+    return 0;
+  };
+  int mode = c->rans_mapping;
+  if (mode == FCS_RANS_MAPPING_AUTO) {
+    if (!v->rans_tuned) {
+      // autotuner: both mappings over the whole volume (identical outputs),
+      // after a one-brick launch of each so code loading is not timed
+      RansArgs one = a;
+      one.nlist = 1;
+      HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_WAVE, c->stream));
+      HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_LANE, c->stream));
+      float tw = 0, tl = 0;
+      if (int rc = timed(FCS_RANS_WAVE, &tw)) return rc;
+      if (int rc = timed(FCS_RANS_LANE, &tl)) return rc;
+      c->timing.tune_wave_ms = tw;
+      c->timing.tune_lane_ms = tl;
+      v->rans_tuned = tw < tl ? FCS_RANS_WAVE : FCS_RANS_LANE;
+    }
+    mode = v->rans_tuned;
+  }
+  float ms = 0;
+  if (int rc = timed(mode, &ms)) return rc;
+  c->timing.rans_mapping = (uint32_t)mode;
   std::vector<RansResult> rr(v->nb);
   HIP_TRY(c, hipMemcpyAsync(rr.data(), v->d_rres, rr.size() * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
   c->timing.rans_ms = ms;
   c->timing.rans_symbols = v->symbols;
   c->timing.rans_bytes = v->compressed;
   bool bad = false;
   for (uint32_t i = 0; i < v->nb; ++i) {
     v->rans_status[i] = rr[i].status;
-    v->levels_ok[i] = (uint8_t)rr[i].levels_ok;
     if (out_status) out_status[i] = rr[i].status;
     bad |= rr[i].status != 0;
   }
@@ -479,6 +524,7 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
   }
   if (int rc = ensure_cache(v)) return rc;
   std::vector<DecodeReq> reqs;
+  std::vector<Alloc> deferred;
   std::vector<int64_t> req_of(n, -1);
   std::vector<uint32_t> st(n, 0);
   for (uint32_t i = 0; i < n; ++i) {
@@ -493,7 +539,7 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
       if (out_off) out_off[i] = v->res_alloc[bi].off + (v->res_pyr[bi] ? fcs_pyramid_level_offset(t) : 0);
       continue;
     }
-    cache_release(v, bi);
+    cache_release(v, bi, &deferred);
     uint64_t off;
     const uint64_t sz = 1ull << (3 * t);
     if (!cache_alloc(v, sz, &off)) {
@@ -509,7 +555,9 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
     if (out_off) out_off[i] = off;
   }
   std::vector<uint32_t> rs;
-  if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
+  const int rc_launch = run_decode(v, reqs, rs, nullptr);
+  cache_free_deferred(v, deferred);
+  if (rc_launch) return rc_launch;
   bool bad = false;
   for (uint32_t i = 0; i < n; ++i) {
     if (req_of[i] >= 0) st[i] |= rs[req_of[i]];
@@ -545,8 +593,20 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
   v->bump = per * v->nb;
   std::vector<uint32_t> rs;
   if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
-  for (uint32_t s : rs)
-    if (s) return fail(c, FCS_E_CORRUPT, "one or more bricks failed decoding");
+  uint32_t bad = 0, first = ~0u, any = 0;
+  for (uint32_t i = 0; i < v->nb; ++i)
+    if (rs[i]) {
+      ++bad;
+      any |= rs[i];
+      if (first == ~0u) first = i;
+    }
+  if (bad)
+    return fail(c, FCS_E_CORRUPT,
+                std::to_string(bad) + " bricks failed decoding (status bits 0x" + [&] {
+                  char b[16];
+                  snprintf(b, sizeof b, "%x", any);
+                  return std::string(b);
+                }() + ", first brick " + std::to_string(first) + ")");
   return 0;
 }
 

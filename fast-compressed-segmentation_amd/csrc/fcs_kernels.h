@@ -5,6 +5,11 @@
 
 #include "fcs_device.h"
 
+// rANS mapping of bricks to the machine
+#define FCS_RANS_AUTO 0
+#define FCS_RANS_LANE 1  // one lane per brick
+#define FCS_RANS_WAVE 2  // one wave per brick
+
 struct RansArgs {
   const DevBrick* bricks;
   const uint8_t* stream;
@@ -14,7 +19,6 @@ struct RansArgs {
   uint32_t freq[16];
   uint32_t cum[17];
   uint32_t* nib;
-  LevelEntry* lvl;
   RansResult* res;
 };
 
@@ -22,8 +26,6 @@ template <typename LT>
 struct OpsArgs {
   const DevBrick* bricks;
   const uint32_t* nib;
-  const LevelEntry* lvl;
-  const RansResult* rres;
   const LT* palette;
   const DecodeReq* reqs;
   LT* out;
@@ -67,7 +69,7 @@ struct MarchArgs {
   uint32_t brick_size, bricks_per_axis;
 };
 
-hipError_t fcs_launch_rans(const RansArgs& a, hipStream_t s);
+hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s);
 size_t fcs_ops_lds_bytes(uint32_t L);
 template <typename LT>
 hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t grid, hipStream_t s);

@@ -1,0 +1,224 @@
+// DDA marcher for gfx950: one ray per lane over the brick cache.
+#include <hip/hip_runtime.h>
+
+#include "fcs_common.h"
+#include "fcs_device.h"
+#include "fcs_kernels.h"
+
+namespace {
+
+// ====================================================================== DDA
+// One ray per lane, rayTrace.wgsl:565-892 up to the first hit, FP32 in the
+// WGSL operation order (built with -ffp-contract=off, IEEE div/sqrt).
+// A zero direction component has tMax = tDelta = +inf and no slab test.
+
+__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float biasf(float t) { return fmx(t * 1e-6f, 1e-5f); }
+
+__device__ __forceinline__ float advance3(int& v0, int& v1, int& v2, float& m0, float& m1, float& m2,
+                                          float d0, float d1, float d2, int s0, int s1, int s2) {
+  float t;
+  if (m0 < m1 && m0 < m2) { t = m0; m0 += d0; v0 += s0; }
+  else if (m1 < m2) { t = m1; m1 += d1; v1 += s1; }
+  else { t = m2; m2 += d2; v2 += s2; }
+  return t;
+}
+
+template <typename LT>
+__global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
+  const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
+  const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= a.w || py >= a.h) return;
+  const size_t pix = (size_t)py * a.w + px;
+  LT hit_label = 0;
+  uint64_t hit_vox = ~0ull;
+  float hit_t = -1.0f;
+  const int b = (int)a.brick_size;
+  const int B = (int)a.bricks_per_axis;
+  const float G = a.grid;
+
+  float ndc_x = ((float)px + 0.5f) / a.wf;
+  ndc_x = ndc_x * 2.0f;
+  ndc_x = ndc_x - 1.0f;
+  float ndc_y = ((float)py + 0.5f) / a.hf;
+  ndc_y = ndc_y * 2.0f;
+  ndc_y = 1.0f - ndc_y;
+  float o[3], d[3], inv[3];
+  bool zero[3];
+  if (a.cam.ortho_half_height > 0.0f) {
+    for (int k = 0; k < 3; ++k) {
+      float ox = a.cam.right[k] * ndc_x;
+      ox = ox * a.aspect;
+      ox = ox * a.cam.ortho_half_height;
+      float oy = a.cam.up[k] * ndc_y;
+      oy = oy * a.cam.ortho_half_height;
+      o[k] = (a.cam.position[k] + ox) + oy;
+      d[k] = a.cam.forward[k];
+    }
+  } else {
+    float v[3];
+    for (int k = 0; k < 3; ++k) {
+      float rt = a.cam.right[k] * ndc_x;
+      rt = rt * a.aspect;
+      rt = rt * a.cam.tan_half_fov;
+      float ut = a.cam.up[k] * ndc_y;
+      ut = ut * a.cam.tan_half_fov;
+      v[k] = (a.cam.forward[k] + rt) + ut;
+      o[k] = a.cam.position[k];
+    }
+    float dd = v[0] * v[0];
+    dd = dd + v[1] * v[1];
+    dd = dd + v[2] * v[2];
+    const float len = sqrtf(dd);
+    for (int k = 0; k < 3; ++k) d[k] = v[k] / len;
+  }
+  for (int k = 0; k < 3; ++k) {
+    zero[k] = d[k] == 0.0f;
+    const float s = d[k] > 0.0f ? 1.0f : (d[k] < 0.0f ? -1.0f : 0.0f);
+    inv[k] = zero[k] ? __builtin_inff() : s / fmx(fabsf(d[k]), 1e-6f);
+  }
+  // slab test (rayTrace.wgsl:457-471)
+  float tmin = -__builtin_inff(), tmax = __builtin_inff();
+  bool miss = false;
+  for (int k = 0; k < 3; ++k) {
+    if (zero[k]) {
+      if (o[k] < 0.0f || o[k] > G) miss = true;
+      continue;
+    }
+    const float t0 = (0.0f - o[k]) * inv[k];
+    const float t1 = (G - o[k]) * inv[k];
+    tmin = fmx(tmin, fmn(t0, t1));
+    tmax = fmn(tmax, fmx(t0, t1));
+  }
+  if (!miss && !(tmax < 0.0f || tmin > tmax)) {
+    const float tEnter = fmx(tmin, 0.0f), tExit = tmax;
+    int step[3], brick[3];
+    float btmax[3], btdelta[3];
+    {
+      const float ts = tEnter + biasf(tEnter);
+      for (int k = 0; k < 3; ++k) {
+        const float pos = o[k] + d[k] * ts;
+        brick[k] = (int)floorf(pos) / b;
+        step[k] = d[k] > 0.0f ? 1 : (d[k] < 0.0f ? -1 : 0);
+        const float bf = (float)brick[k] * (float)b;
+        const float nb = d[k] > 0.0f ? bf + (float)b : bf;
+        btmax[k] = zero[k] ? __builtin_inff() : (nb - o[k]) * inv[k];
+        btdelta[k] = zero[k] ? __builtin_inff() : fabsf((float)b * inv[k]);
+      }
+    }
+    uint32_t maxb = 6;
+    for (int k = 0; k < 3; ++k) {
+      const int be = (int)(o[k] + d[k] * tEnter) / b;
+      const int bx = (int)(o[k] + d[k] * tExit) / b;
+      maxb += (uint32_t)abs(bx - be);
+    }
+    float brickEntryT = tEnter;
+    for (uint32_t it = 0; it < maxb; ++it) {
+      if (brick[0] < 0 || brick[1] < 0 || brick[2] < 0 || brick[0] >= B || brick[1] >= B || brick[2] >= B) break;
+      const uint32_t bi = fcs_morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
+      const uint32_t meta = a.meta[bi];
+      if (!(meta & FCS_META_EMPTY)) {
+        if (a.visited) a.visited[bi] = 1u;
+        const int bmin0 = brick[0] * b, bmin1 = brick[1] * b, bmin2 = brick[2] * b;
+        const float tEnterBrick = fmx(brickEntryT, tEnter);
+        float tExitBrick = fmn(fmn(btmax[0], btmax[1]), btmax[2]);
+        tExitBrick = fmx(tEnterBrick, tExitBrick - biasf(tExitBrick));
+        int vox[3];
+        float vtmax[3], vtdelta[3];
+        const float ts = tEnterBrick + biasf(tEnterBrick);
+        const int bmin[3] = {bmin0, bmin1, bmin2};
+        for (int k = 0; k < 3; ++k) {
+          const float pos = o[k] + d[k] * ts;
+          vox[k] = (int)floorf(pos);
+          const float nb = d[k] > 0.0f ? (float)(vox[k] + 1) : (float)vox[k];
+          vtmax[k] = zero[k] ? __builtin_inff() : (nb - o[k]) * inv[k];
+          vtdelta[k] = zero[k] ? __builtin_inff() : fabsf(inv[k]);
+        }
+        const uint32_t lod = meta & FCS_META_LOD_MASK;
+        const int lodSize = 1 << lod;
+        const int stride = b / lodSize;
+        for (int k = 0; k < 3; ++k) vox[k] -= bmin[k];
+        int vstep[3] = {step[0], step[1], step[2]};
+        if (stride > 1) {
+          for (int k = 0; k < 3; ++k) {
+            int lvx = (vox[k] / stride) * stride;
+            lvx = lvx < 0 ? 0 : (lvx > b - 1 ? b - 1 : lvx);
+            vox[k] = lvx;
+            vstep[k] = step[k] * stride;  // one LOD cell per step
+            vtdelta[k] = zero[k] ? __builtin_inff() : fabsf(inv[k]) * (float)stride;
+            const float lw = (float)bmin[k] + (float)vox[k];
+            const float nb = d[k] > 0.0f ? lw + (float)stride : lw;
+            vtmax[k] = zero[k] ? __builtin_inff() : (nb - o[k]) * inv[k];
+          }
+        } else {
+          for (int k = 0; k < 3; ++k) vox[k] = vox[k] < 0 ? 0 : (vox[k] > b - 1 ? b - 1 : vox[k]);
+        }
+        uint32_t maxs = 128;
+        for (int k = 0; k < 3; ++k) {
+          const float ve = o[k] + d[k] * tEnterBrick;
+          const float vx = o[k] + d[k] * tExitBrick;
+          const float le = (ve - (float)bmin[k]) / (float)stride;
+          const float lx = (vx - (float)bmin[k]) / (float)stride;
+          maxs += (uint32_t)abs((int)ceilf(lx) - (int)floorf(le));
+        }
+        const bool resident = (meta & FCS_META_RESIDENT) != 0;
+        if (lod != 0 && !resident) {
+          if (a.misses) atomicAdd(a.misses, 1u);
+        } else {
+          const LT* lab = a.cache + a.offs[bi];
+          const float lodScale = (float)lodSize / (float)b;
+          float tcur = tEnterBrick;
+          for (uint32_t s = 0; s < maxs; ++s) {
+            if (vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < b && vox[1] < b && vox[2] < b) {
+              LT v;
+              if (lod == 0) {
+                v = a.pal0[bi];
+              } else {
+                uint32_t cc[3];
+                for (int k = 0; k < 3; ++k) {
+                  float sc = (float)vox[k] * lodScale;
+                  sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
+                  cc[k] = (uint32_t)sc;
+                }
+                v = lab[fcs_morton3(cc[0], cc[1], cc[2])];
+              }
+              if (v != 0) {
+                hit_label = v;
+                const uint64_t Gi = (uint64_t)B * (uint64_t)b;
+                hit_vox = (uint64_t)(bmin[0] + vox[0]) +
+                          Gi * ((uint64_t)(bmin[1] + vox[1]) + Gi * (uint64_t)(bmin[2] + vox[2]));
+                hit_t = tcur;
+                break;
+              }
+            }
+            const float tt = advance3(vox[0], vox[1], vox[2], vtmax[0], vtmax[1], vtmax[2], vtdelta[0],
+                                      vtdelta[1], vtdelta[2], vstep[0], vstep[1], vstep[2]);
+            tcur = tt;
+            if (tt > tExitBrick) break;
+          }
+          if (hit_t >= 0.0f) break;
+        }
+      }
+      const float tt = advance3(brick[0], brick[1], brick[2], btmax[0], btmax[1], btmax[2], btdelta[0],
+                                btdelta[1], btdelta[2], step[0], step[1], step[2]);
+      brickEntryT = tt + biasf(tt);
+      if (tt > tExit) break;
+    }
+  }
+  if (a.out_label) a.out_label[pix] = hit_label;
+  if (a.out_voxel) a.out_voxel[pix] = hit_vox;
+  if (a.out_t) a.out_t[pix] = hit_t;
+}
+
+}  // namespace
+
+template <typename LT>
+hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {
+  const dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
+  hipLaunchKernelGGL(march_kernel<LT>, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template hipError_t fcs_launch_march<uint32_t>(const MarchArgs<uint32_t>&, hipStream_t);
+template hipError_t fcs_launch_march<uint64_t>(const MarchArgs<uint64_t>&, hipStream_t);

@@ -1,0 +1,284 @@
+// rANS stage for gfx950: every brick's rANS-coded op stream -> its nibble
+// stream in HBM (8 nibbles per u32, nibble k of a brick at bits 4*(k&7) of
+// word k>>3). Restates decodeRansStream (brickDecoding.cpp:126-188):
+//   x = s & 4095; symbol = CDF^-1(x); s = f*(s>>12) + x - c;
+//   while (s < 2^23 && bytes left) s = (s << 8) | next byte (read backwards)
+// The initial state is the last 4 bytes (byte E-1 most significant).
+// After the pad symbol of an odd count, a well-formed stream is back at
+// state 2^23 with every byte consumed; violations set FCS_ST_RANS_STATE.
+//
+// Two mappings of bricks to the machine (picked by the runtime):
+//   rans_wave_kernel: one wave per brick. The stream is staged in an LDS ring
+//     (coalesced 64-lane loads, next half-ring prefetched in VGPRs), the rANS
+//     state is wave-uniform (SGPRs, scalar ALU), the symbol search is one
+//     vector compare of x against a per-lane CDF + ballot + popcount, and
+//     nibble word k of a 64-word group is kept in lane k, stored 256 B at a time.
+//     Lowest latency per brick: for volumes with few, long bricks.
+//   rans_lane_kernel: one lane per brick, 64 bricks per wave, symbol table in
+//     LDS, stream fetched 128 bytes per lane at a time. Highest throughput:
+//     for volumes with many bricks.
+#include <hip/hip_runtime.h>
+
+#include "fcs_common.h"
+#include "fcs_device.h"
+#include "fcs_kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------- wave per brick
+#define RING_WORDS 4096u
+#define RING_HALF 2048u
+#define RING_PEND (RING_HALF / 64u)
+
+__global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
+  __shared__ uint32_t ring[RING_WORDS];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t b = a.list ? a.list[blockIdx.x] : blockIdx.x;
+  const DevBrick B = a.bricks[b];
+  const uint32_t E = B.E, nsym = B.nsym;
+  if (E < 4 || nsym == 0) {
+    if (lane == 0) a.res[b] = RansResult{FCS_ST_RANS_STATE | FCS_ST_STREAM_OVERRUN, 0u};
+    return;
+  }
+  uint32_t* nibw = a.nib + B.nib_off;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(a.stream + B.str_off);
+  // per-lane CDF: lane k < 16 holds cum[k] and freq[k]
+  const uint32_t my_cum = lane < 16 ? a.cum[lane] : 0xFFFFFFFFu;
+  const uint32_t my_f = lane < 16 ? a.freq[lane] : 0u;
+
+  const int32_t nwords = (int32_t)((E + 3) >> 2);
+  int32_t lo = nwords > (int32_t)RING_WORDS ? nwords - (int32_t)RING_WORDS : 0;
+  for (int32_t w = lo + (int32_t)lane; w < nwords; w += 64) ring[w & (RING_WORDS - 1)] = words[w];
+  uint32_t pend[RING_PEND];
+#pragma unroll
+  for (uint32_t i = 0; i < RING_PEND; ++i) {
+    const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
+    pend[i] = w >= 0 ? words[w] : 0u;
+  }
+  __syncthreads();
+
+  uint32_t pos = E;  // the next byte read is pos - 1
+  int32_t pw = -1;   // word index held in curw
+  uint32_t curw = 0;
+  auto next_byte = [&]() -> uint32_t {
+    --pos;
+    const int32_t wi = (int32_t)(pos >> 2);
+    if (wi != pw) {
+      if (lo > 0 && wi < lo + (int32_t)RING_HALF) {
+        // words [lo + HALF, lo + RING) are consumed: their slots take the
+        // prefetched half [lo - HALF, lo); then prefetch the half below
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < RING_PEND; ++i) {
+          const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
+          if (w >= 0) ring[w & (RING_WORDS - 1)] = pend[i];
+        }
+        lo = lo > (int32_t)RING_HALF ? lo - (int32_t)RING_HALF : 0;
+#pragma unroll
+        for (uint32_t i = 0; i < RING_PEND; ++i) {
+          const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
+          pend[i] = w >= 0 ? words[w] : 0u;
+        }
+        __syncthreads();
+      }
+      pw = wi;
+      curw = __builtin_amdgcn_readfirstlane(ring[wi & (RING_WORDS - 1)]);
+    }
+    return (curw >> ((pos & 3u) * 8u)) & 0xFFu;
+  };
+
+  uint32_t state = next_byte() << 24;
+  state |= next_byte() << 16;
+  state |= next_byte() << 8;
+  state |= next_byte();
+
+  uint32_t outv = 0, nw = 0;
+  for (uint32_t g = 0; g < nsym; ++g) {
+    const uint32_t x = state & (FCS_RANS_M - 1);
+    const uint32_t s = (uint32_t)__popcll(__ballot(x >= my_cum)) - 1u;
+    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)s);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_cum, (int)s);
+    state = f * (state >> FCS_RANS_LOGM) + x - c;
+    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | next_byte();
+    nw |= s << ((g & 7u) * 4u);
+    if ((g & 7u) == 7u) {
+      const uint32_t wl = (g >> 3) & 63u;
+      outv = lane == wl ? nw : outv;  // word wl of the 64-word group lives in lane wl
+      nw = 0;
+      if (wl == 63u) nibw[(g >> 9) * 64u + lane] = outv;
+    }
+  }
+  const uint32_t total_words = (nsym + 7) >> 3;
+  if (nsym & 7u) outv = lane == ((nsym >> 3) & 63u) ? nw : outv;
+  const uint32_t stored = (nsym >> 9) * 64u;
+  if (lane < total_words - stored) nibw[stored + lane] = outv;
+
+  uint32_t status = 0;
+  if (nsym & 1u) {
+    const uint32_t x = state & (FCS_RANS_M - 1);
+    const uint32_t s = (uint32_t)__popcll(__ballot(x >= my_cum)) - 1u;
+    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)s);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_cum, (int)s);
+    state = f * (state >> FCS_RANS_LOGM) + x - c;
+    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | next_byte();
+    if (s != 0) status |= FCS_ST_RANS_STATE;
+  }
+  if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  if (lane == 0) a.res[b] = RansResult{status, 0u};
+}
+
+// ---------------------------------------------------------- lane per brick
+// Backward byte reader: 128 bytes (8 blocks of 16) are fetched per refill and
+// consumed through register shifts, so a lane waits for memory once per 128
+// stream bytes (~300 symbols) and the other waves of the SIMD fill that wait.
+struct LaneReader {
+  const uint4* base;
+  int32_t q;       // next block index to fetch (decreasing)
+  uint32_t cur;    // current word, next byte in bits 31..24
+  uint32_t nb;     // bytes left in cur
+  uint32_t w2, w1, w0, nw;  // lower words of the current block (w2 next), count
+  uint32_t nblk;   // blocks left in b0..b7
+  uint4 b0, b1, b2, b3, b4, b5, b6, b7;
+  __device__ __forceinline__ static uint32_t pick(const uint4& v, uint32_t j) {
+    return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+  }
+  __device__ __forceinline__ uint4 blk(int32_t k) const {
+    return k >= 0 ? base[k] : make_uint4(0u, 0u, 0u, 0u);
+  }
+  __device__ __forceinline__ void refill() {
+    b0 = blk(q);
+    b1 = blk(q - 1);
+    b2 = blk(q - 2);
+    b3 = blk(q - 3);
+    b4 = blk(q - 4);
+    b5 = blk(q - 5);
+    b6 = blk(q - 6);
+    b7 = blk(q - 7);
+    q -= 8;
+    nblk = 8;
+  }
+  __device__ __forceinline__ void shift() {
+    b0 = b1;
+    b1 = b2;
+    b2 = b3;
+    b3 = b4;
+    b4 = b5;
+    b5 = b6;
+    b6 = b7;
+    --nblk;
+  }
+  __device__ __forceinline__ void init(const uint8_t* s, uint32_t E) {
+    base = reinterpret_cast<const uint4*>(s);
+    q = (int32_t)((E - 1) >> 4);
+    refill();
+    const uint32_t j = ((E - 1) >> 2) & 3u;    // word of byte E-1 in its block
+    const uint32_t skip = 3u - ((E - 1) & 3u); // bytes of that word beyond E
+    cur = pick(b0, j) << (8u * skip);
+    nb = 4u - skip;
+    w2 = j >= 1 ? pick(b0, j - 1) : 0u;
+    w1 = j >= 2 ? pick(b0, j - 2) : 0u;
+    w0 = j >= 3 ? b0.x : 0u;
+    nw = j;
+    shift();
+  }
+  __device__ __forceinline__ uint32_t byte() {
+    if (nb == 0) {
+      if (nw == 0) {
+        if (nblk == 0) refill();
+        cur = b0.w;
+        w2 = b0.z;
+        w1 = b0.y;
+        w0 = b0.x;
+        nw = 3;
+        shift();
+      } else {
+        cur = w2;
+        w2 = w1;
+        w1 = w0;
+        --nw;
+      }
+      nb = 4;
+    }
+    const uint32_t r = cur >> 24;
+    cur <<= 8;
+    --nb;
+    return r;
+  }
+};
+
+__global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
+  __shared__ uint32_t table[FCS_RANS_M];
+  for (uint32_t x = threadIdx.x; x < FCS_RANS_M; x += blockDim.x) {
+    uint32_t s = 0;
+    while (s + 1 < 16 && x >= a.cum[s + 1]) ++s;
+    table[x] = (s << 25) | (a.freq[s] << 12) | a.cum[s];
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nlist) return;
+  const uint32_t b = a.list ? a.list[i] : i;
+  const DevBrick B = a.bricks[b];
+  const uint32_t E = B.E, nsym = B.nsym;
+  if (E < 4 || nsym == 0) {
+    a.res[b] = RansResult{FCS_ST_RANS_STATE | FCS_ST_STREAM_OVERRUN, 0u};
+    return;
+  }
+  uint32_t* nibw = a.nib + B.nib_off;
+  LaneReader rd;
+  rd.init(a.stream + B.str_off, E);
+  uint32_t pos = E;
+  uint32_t state = 0;
+  for (int k = 0; k < 4; ++k) {
+    state = (state << 8) | rd.byte();
+    --pos;
+  }
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, cw = 0;
+  for (uint32_t g = 0; g < nsym; ++g) {
+    const uint32_t x = state & (FCS_RANS_M - 1);
+    const uint32_t e = table[x];
+    state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
+    while (state < FCS_RANS_L && pos > 0) {
+      state = (state << 8) | rd.byte();
+      --pos;
+    }
+    cw |= (e >> 25) << ((g & 7u) * 4u);
+    if ((g & 7u) == 7u) {
+      const uint32_t wi = (g >> 3) & 3u;
+      if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
+      cw = 0;
+      if (wi == 3) reinterpret_cast<uint4*>(nibw)[g >> 5] = make_uint4(w0, w1, w2, w3);
+    }
+  }
+  if (nsym & 31u) {
+    const uint32_t wi = (nsym >> 3) & 3u;
+    if (nsym & 7u) {
+      if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
+    }
+    reinterpret_cast<uint4*>(nibw)[nsym >> 5] = make_uint4(w0, w1, w2, w3);
+  }
+  uint32_t status = 0;
+  if (nsym & 1u) {
+    const uint32_t x = state & (FCS_RANS_M - 1);
+    const uint32_t e = table[x];
+    state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
+    while (state < FCS_RANS_L && pos > 0) {
+      state = (state << 8) | rd.byte();
+      --pos;
+    }
+    if ((e >> 25) != 0) status |= FCS_ST_RANS_STATE;
+  }
+  if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  a.res[b] = RansResult{status, 0u};
+}
+
+}  // namespace
+
+hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s) {
+  if (a.nlist == 0) return hipSuccess;
+  if (mode == FCS_RANS_WAVE) {
+    hipLaunchKernelGGL(rans_wave_kernel, dim3(a.nlist), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(rans_lane_kernel, dim3((a.nlist + 255) / 256), dim3(256), 0, s, a);
+  }
+  return hipGetLastError();
+}

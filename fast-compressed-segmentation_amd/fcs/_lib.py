@@ -94,12 +94,18 @@ class TimingStruct(ctypes.Structure):
         ("rans_ms", ctypes.c_float),
         ("decode_ms", ctypes.c_float),
         ("march_ms", ctypes.c_float),
-        ("pad", ctypes.c_float),
+        ("rans_mapping", ctypes.c_uint32),
         ("decoded_voxels", ctypes.c_uint64),
         ("decode_bytes", ctypes.c_uint64),
         ("rans_symbols", ctypes.c_uint64),
         ("rans_bytes", ctypes.c_uint64),
+        ("tune_lane_ms", ctypes.c_float),
+        ("tune_wave_ms", ctypes.c_float),
     ]
+
+
+RANS_AUTO, RANS_LANE, RANS_WAVE = 0, 1, 2
+OPT_RANS_MAPPING = 1
 
 
 class GenParams(ctypes.Structure):
@@ -133,6 +139,7 @@ _sig("fcs_destroy", None, c_vp)
 _sig("fcs_last_error", ctypes.c_char_p, c_vp)
 _sig("fcs_sync", ctypes.c_int, c_vp)
 _sig("fcs_get_timing", ctypes.c_int, c_vp, ctypes.POINTER(TimingStruct))
+_sig("fcs_set_option", ctypes.c_int, c_vp, ctypes.c_int, ctypes.c_int64)
 _sig("fcs_load_csdb", ctypes.c_int, c_vp, c_vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(c_vp))
 _sig("fcs_volume_free", None, c_vp)
 _sig("fcs_volume_get_info", ctypes.c_int, c_vp, ctypes.POINTER(VolumeInfo))

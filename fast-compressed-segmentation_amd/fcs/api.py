@@ -89,7 +89,11 @@ class Context:
     def timing(self):
         t = TimingStruct()
         check(lib.fcs_get_timing(self.h, ctypes.byref(t)), self.h)
-        return {f: getattr(t, f) for f, _ in TimingStruct._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in TimingStruct._fields_}
+
+    def set_rans_mapping(self, mapping):
+        """0 = autotuned, 1 = one lane per brick, 2 = one wave per brick."""
+        check(lib.fcs_set_option(self.h, 1, int(mapping)), self.h)
 
     def sync(self):
         check(lib.fcs_sync(self.h), self.h)

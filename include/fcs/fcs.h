@@ -111,12 +111,20 @@ typedef struct {
   float rans_ms;          /* last fcs_decode_rans */
   float decode_ms;        /* last fcs_request_bricks / fcs_decode_all */
   float march_ms;         /* last fcs_render march */
-  float pad;
+  uint32_t rans_mapping;  /* FCS_RANS_MAPPING_* used by the last fcs_decode_rans */
   uint64_t decoded_voxels;  /* labels written by the last decode */
   uint64_t decode_bytes;    /* algorithmic bytes of the last decode */
   uint64_t rans_symbols;    /* symbols decoded by the last fcs_decode_rans */
   uint64_t rans_bytes;      /* compressed bytes consumed by the last fcs_decode_rans */
+  float tune_lane_ms;     /* autotuner: whole-volume rANS time, one lane per brick */
+  float tune_wave_ms;     /* autotuner: whole-volume rANS time, one wave per brick */
 } fcs_timing;
+
+/* Options (fcs_set_option). */
+#define FCS_OPT_RANS_MAPPING 1 /* FCS_RANS_MAPPING_*; default AUTO */
+#define FCS_RANS_MAPPING_AUTO 0 /* measure both on the volume's first rANS pass, keep the faster */
+#define FCS_RANS_MAPPING_LANE 1 /* one lane per brick (many bricks) */
+#define FCS_RANS_MAPPING_WAVE 2 /* one wave per brick (few, long bricks) */
 
 int fcs_create(int hip_device, fcs_ctx** out);
 void fcs_destroy(fcs_ctx* ctx);
@@ -143,7 +151,10 @@ int fcs_cache_clear(fcs_volume* vol);
 /* Decode n (brick, LOD) requests into the brick cache (one batched launch).
  * Bricks already resident at that LOD are not decoded again.
  * out_cache_offset (nullable): voxel offset of each request's labels in the pool.
- * out_status (nullable): per-request FCS_ST_* bits. */
+ * out_status (nullable): per-request FCS_ST_* bits.
+ * A brick is resident at one LOD (or as a pyramid) at a time: when a call
+ * requests a brick at several LODs, the last request wins and the regions of
+ * the earlier ones stay valid until the next call. */
 int fcs_request_bricks(fcs_volume* vol, const uint32_t* brick, const uint8_t* lod, uint32_t n,
                        uint64_t* out_cache_offset, uint32_t* out_status);
 
@@ -174,6 +185,8 @@ int fcs_camera_orbit(const fcs_volume* vol, double yaw, double pitch, double rad
                      fcs_camera* out);
 
 int fcs_get_timing(const fcs_ctx* ctx, fcs_timing* out);
+/* No reference counterpart: runtime knobs (FCS_OPT_*). */
+int fcs_set_option(fcs_ctx* ctx, int option, int64_t value);
 int fcs_sync(fcs_ctx* ctx);
 
 #ifdef __cplusplus

@@ -1,0 +1,30 @@
+'use strict';
+// Drives the Node.js module the way web/app.js drives the reference
+// (load -> decodeAllRansBricks -> decode -> render) and writes the results
+// as raw little-endian files for the Python parity test to compare.
+const fs = require('fs');
+const path = require('path');
+const fcs = require(path.join(__dirname, '..', '..', 'fast-compressed-segmentation_amd', 'node'));
+
+const [csdbPath, outDir, brickSize] = process.argv.slice(2);
+const buf = fs.readFileSync(csdbPath);
+const ds = fcs.loadDatasetFromArrayBuffer(new Uint8Array(buf.buffer, buf.byteOffset, buf.length), Number(brickSize));
+const st = fcs.decodeAllRansBricks(ds);
+fs.writeFileSync(path.join(outDir, 'rans_status.bin'), Buffer.from(st.buffer));
+const h = ds.header;
+// every brick at every LOD through readBrick
+const parts = [];
+for (let b = 0; b < h.numBricks; b++) {
+  for (let t = 0; t <= h.levels; t++) parts.push(Buffer.from(fcs.readBrick(ds, b, t).buffer));
+}
+fs.writeFileSync(path.join(outDir, 'bricks.bin'), Buffer.concat(parts));
+const g = h.gridSize;
+const cam = { position: [g / 2, g / 2, -1], forward: [0, 0, 1], right: [1, 0, 0], up: [0, -1, 0],
+              tanHalfFov: 0, orthoHalfHeight: g / 2 };
+const lods = new Uint8Array(h.numBricks).fill(h.levels);
+const f = fcs.render(ds, cam, 256, 256, lods);
+fs.writeFileSync(path.join(outDir, 'label.bin'), Buffer.from(f.label.buffer));
+fs.writeFileSync(path.join(outDir, 'voxel.bin'), Buffer.from(f.voxel.buffer));
+let threw = false;
+try { fcs.loadDatasetFromArrayBuffer(new Uint8Array(64)); } catch (e) { threw = /Invalid file format/.test(e.message); }
+console.log(JSON.stringify({ header: h, badMagicThrows: threw, marchMs: f.marchMs }));
