@@ -49,6 +49,7 @@ struct fcs_volume {
   void* d_pal0 = nullptr;
   uint32_t* d_nib = nullptr;
   RansResult* d_rres = nullptr;
+  uint32_t* d_rtab = nullptr;  // x -> (symbol << 25) | (freq << 12) | (x - cum)
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
   int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
@@ -260,7 +261,7 @@ int fcs_sync(fcs_ctx* c) {
 int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
   if (!c) return FCS_E_INVALID;
   if (option == FCS_OPT_RANS_MAPPING) {
-    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE) return fail(c, FCS_E_INVALID, "bad mapping");
+    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_BALLOT) return fail(c, FCS_E_INVALID, "bad mapping");
     c->rans_mapping = (int)value;
     return 0;
   }
@@ -392,6 +393,15 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     return fail(c, FCS_E_NOMEM, "device allocation failed");
   }
   (void)hipMemset(v->d_nib, 0, (nib_total + 8) * 4);
+  {
+    std::vector<uint32_t> tab(FCS_RANS_M);
+    for (uint32_t x = 0; x < FCS_RANS_M; ++x) {
+      uint32_t sym = 0;
+      while (sym + 1 < 16 && x >= v->cum[sym + 1]) ++sym;
+      tab[x] = (sym << 25) | (v->freq[sym] << 12) | (x - v->cum[sym]);
+    }
+    if (int rc2 = dev_upload(c, &v->d_rtab, tab.data(), tab.size() * 4)) { fcs_volume_free(v); return rc2; }
+  }
   v->res_lod.assign(v->nb, -1);
   v->res_pyr.assign(v->nb, 0);
   v->res_alloc.assign(v->nb, Alloc{});
@@ -404,7 +414,7 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
 void fcs_volume_free(fcs_volume* v) {
   if (!v) return;
   if (v->ctx) (void)hipSetDevice(v->ctx->device);
-  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter,
+  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab,
                   v->d_cache, v->d_req, v->d_req_status, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel,
                   v->d_ft, v->d_visited, v->d_misses};
   for (void* q : ptrs)
@@ -446,6 +456,7 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   memcpy(a.cum, v->cum, sizeof a.cum);
   a.nib = v->d_nib;
   a.res = v->d_rres;
+  a.table = v->d_rtab;
   auto timed = [&](int mode, float* ms) -> int {
     HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(c, fcs_launch_rans(a, mode, c->stream));
@@ -463,12 +474,14 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
       one.nlist = 1;
       HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_WAVE, c->stream));
       HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_LANE, c->stream));
-      float tw = 0, tl = 0;
+      float tw = 0, tl = 0, tb = 0;
       if (int rc = timed(FCS_RANS_WAVE, &tw)) return rc;
+      if (int rc = timed(FCS_RANS_WAVE_BALLOT, &tb)) return rc;
       if (int rc = timed(FCS_RANS_LANE, &tl)) return rc;
       c->timing.tune_wave_ms = tw;
+      c->timing.tune_ballot_ms = tb;
       c->timing.tune_lane_ms = tl;
-      v->rans_tuned = tw < tl ? FCS_RANS_WAVE : FCS_RANS_LANE;
+      v->rans_tuned = (tw <= tb && tw <= tl) ? FCS_RANS_WAVE : (tb <= tl ? FCS_RANS_WAVE_BALLOT : FCS_RANS_LANE);
     }
     mode = v->rans_tuned;
   }

@@ -8,7 +8,8 @@
 // rANS mapping of bricks to the machine
 #define FCS_RANS_AUTO 0
 #define FCS_RANS_LANE 1  // one lane per brick
-#define FCS_RANS_WAVE 2  // one wave per brick
+#define FCS_RANS_WAVE 2  // one wave per brick, scalar-cache decode table
+#define FCS_RANS_WAVE_BALLOT 3  // one wave per brick, ballot over a per-lane CDF
 
 struct RansArgs {
   const DevBrick* bricks;
@@ -20,6 +21,7 @@ struct RansArgs {
   uint32_t cum[17];
   uint32_t* nib;
   RansResult* res;
+  const uint32_t* table;  // nullable: x -> (symbol << 25) | (freq << 12) | (x - cum)
 };
 
 template <typename LT>

@@ -16,12 +16,20 @@ __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float biasf(float t) { return fmx(t * 1e-6f, 1e-5f); }
 
+// advanceDDA / advanceBrickDDA (rayTrace.wgsl:501-563): ties prefer z, then y.
+// Written with selects so the axis choice never becomes an indexed access.
 __device__ __forceinline__ float advance3(int& v0, int& v1, int& v2, float& m0, float& m1, float& m2,
                                           float d0, float d1, float d2, int s0, int s1, int s2) {
-  float t;
-  if (m0 < m1 && m0 < m2) { t = m0; m0 += d0; v0 += s0; }
-  else if (m1 < m2) { t = m1; m1 += d1; v1 += s1; }
-  else { t = m2; m2 += d2; v2 += s2; }
+  const bool ax = m0 < m1 && m0 < m2;
+  const bool ay = !ax && m1 < m2;
+  const bool az = !ax && !ay;
+  const float t = ax ? m0 : (ay ? m1 : m2);
+  m0 = ax ? m0 + d0 : m0;
+  m1 = ay ? m1 + d1 : m1;
+  m2 = az ? m2 + d2 : m2;
+  v0 += ax ? s0 : 0;
+  v1 += ay ? s1 : 0;
+  v2 += az ? s2 : 0;
   return t;
 }
 
@@ -47,7 +55,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   float o[3], d[3], inv[3];
   bool zero[3];
   if (a.cam.ortho_half_height > 0.0f) {
-    for (int k = 0; k < 3; ++k) {
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) {
       float ox = a.cam.right[k] * ndc_x;
       ox = ox * a.aspect;
       ox = ox * a.cam.ortho_half_height;
@@ -58,7 +66,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
     }
   } else {
     float v[3];
-    for (int k = 0; k < 3; ++k) {
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) {
       float rt = a.cam.right[k] * ndc_x;
       rt = rt * a.aspect;
       rt = rt * a.cam.tan_half_fov;
@@ -71,9 +79,9 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
     dd = dd + v[1] * v[1];
     dd = dd + v[2] * v[2];
     const float len = sqrtf(dd);
-    for (int k = 0; k < 3; ++k) d[k] = v[k] / len;
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) d[k] = v[k] / len;
   }
-  for (int k = 0; k < 3; ++k) {
+  _Pragma("unroll") for (int k = 0; k < 3; ++k) {
     zero[k] = d[k] == 0.0f;
     const float s = d[k] > 0.0f ? 1.0f : (d[k] < 0.0f ? -1.0f : 0.0f);
     inv[k] = zero[k] ? __builtin_inff() : s / fmx(fabsf(d[k]), 1e-6f);
@@ -81,7 +89,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   // slab test (rayTrace.wgsl:457-471)
   float tmin = -__builtin_inff(), tmax = __builtin_inff();
   bool miss = false;
-  for (int k = 0; k < 3; ++k) {
+  _Pragma("unroll") for (int k = 0; k < 3; ++k) {
     if (zero[k]) {
       if (o[k] < 0.0f || o[k] > G) miss = true;
       continue;
@@ -97,7 +105,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
     float btmax[3], btdelta[3];
     {
       const float ts = tEnter + biasf(tEnter);
-      for (int k = 0; k < 3; ++k) {
+      _Pragma("unroll") for (int k = 0; k < 3; ++k) {
         const float pos = o[k] + d[k] * ts;
         brick[k] = (int)floorf(pos) / b;
         step[k] = d[k] > 0.0f ? 1 : (d[k] < 0.0f ? -1 : 0);
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       }
     }
     uint32_t maxb = 6;
-    for (int k = 0; k < 3; ++k) {
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) {
       const int be = (int)(o[k] + d[k] * tEnter) / b;
       const int bx = (int)(o[k] + d[k] * tExit) / b;
       maxb += (uint32_t)abs(bx - be);
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
         float vtmax[3], vtdelta[3];
         const float ts = tEnterBrick + biasf(tEnterBrick);
         const int bmin[3] = {bmin0, bmin1, bmin2};
-        for (int k = 0; k < 3; ++k) {
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) {
           const float pos = o[k] + d[k] * ts;
           vox[k] = (int)floorf(pos);
           const float nb = d[k] > 0.0f ? (float)(vox[k] + 1) : (float)vox[k];
@@ -138,10 +146,10 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
         const uint32_t lod = meta & FCS_META_LOD_MASK;
         const int lodSize = 1 << lod;
         const int stride = b / lodSize;
-        for (int k = 0; k < 3; ++k) vox[k] -= bmin[k];
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) vox[k] -= bmin[k];
         int vstep[3] = {step[0], step[1], step[2]};
         if (stride > 1) {
-          for (int k = 0; k < 3; ++k) {
+          _Pragma("unroll") for (int k = 0; k < 3; ++k) {
             int lvx = (vox[k] / stride) * stride;
             lvx = lvx < 0 ? 0 : (lvx > b - 1 ? b - 1 : lvx);
             vox[k] = lvx;
@@ -152,10 +160,10 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
             vtmax[k] = zero[k] ? __builtin_inff() : (nb - o[k]) * inv[k];
           }
         } else {
-          for (int k = 0; k < 3; ++k) vox[k] = vox[k] < 0 ? 0 : (vox[k] > b - 1 ? b - 1 : vox[k]);
+          _Pragma("unroll") for (int k = 0; k < 3; ++k) vox[k] = vox[k] < 0 ? 0 : (vox[k] > b - 1 ? b - 1 : vox[k]);
         }
         uint32_t maxs = 128;
-        for (int k = 0; k < 3; ++k) {
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) {
           const float ve = o[k] + d[k] * tEnterBrick;
           const float vx = o[k] + d[k] * tExitBrick;
           const float le = (ve - (float)bmin[k]) / (float)stride;
@@ -176,7 +184,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
                 v = a.pal0[bi];
               } else {
                 uint32_t cc[3];
-                for (int k = 0; k < 3; ++k) {
+                _Pragma("unroll") for (int k = 0; k < 3; ++k) {
                   float sc = (float)vox[k] * lodScale;
                   sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
                   cc[k] = (uint32_t)sc;

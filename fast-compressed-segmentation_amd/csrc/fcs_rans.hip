@@ -42,9 +42,11 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
   }
   uint32_t* nibw = a.nib + B.nib_off;
   const uint32_t* words = reinterpret_cast<const uint32_t*>(a.stream + B.str_off);
-  // per-lane CDF: lane k < 16 holds cum[k] and freq[k]
-  const uint32_t my_cum = lane < 16 ? a.cum[lane] : 0xFFFFFFFFu;
+  // per-lane CDF: lane k < 15 holds cum[k+1] (so the symbol is the number of
+  // lanes with cum[k+1] <= x), lane k < 16 holds freq[k] and cum[k]
+  const uint32_t my_next = lane < 15 ? a.cum[lane + 1] : 0xFFFFFFFFu;
   const uint32_t my_f = lane < 16 ? a.freq[lane] : 0u;
+  const uint32_t my_c = lane < 16 ? a.cum[lane] : 0u;
 
   const int32_t nwords = (int32_t)((E + 3) >> 2);
   int32_t lo = nwords > (int32_t)RING_WORDS ? nwords - (int32_t)RING_WORDS : 0;
@@ -53,75 +55,101 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
 #pragma unroll
   for (uint32_t i = 0; i < RING_PEND; ++i) {
     const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
-    pend[i] = w >= 0 ? words[w] : 0u;
+    pend[i] = words[w < 0 ? 0 : w];
   }
   __syncthreads();
 
-  uint32_t pos = E;  // the next byte read is pos - 1
-  int32_t pw = -1;   // word index held in curw
-  uint32_t curw = 0;
-  auto next_byte = [&]() -> uint32_t {
-    --pos;
-    const int32_t wi = (int32_t)(pos >> 2);
-    if (wi != pw) {
-      if (lo > 0 && wi < lo + (int32_t)RING_HALF) {
-        // words [lo + HALF, lo + RING) are consumed: their slots take the
-        // prefetched half [lo - HALF, lo); then prefetch the half below
-        __syncthreads();
+  // wave-uniform byte source: cw holds the current word, next byte on top
+  uint32_t pos = E;  // bytes left to read
+  int32_t wnext = (int32_t)((E - 1) >> 2);  // next word to fetch from the ring
+  uint32_t cw, nbb;
+  auto fetch_word = [&]() {
+    if (lo > 0 && wnext < lo + (int32_t)RING_HALF) {
+      // words [lo + HALF, lo + RING) are consumed: their slots take the
+      // prefetched half [lo - HALF, lo); then prefetch the half below
+      __syncthreads();
 #pragma unroll
-        for (uint32_t i = 0; i < RING_PEND; ++i) {
-          const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
-          if (w >= 0) ring[w & (RING_WORDS - 1)] = pend[i];
-        }
-        lo = lo > (int32_t)RING_HALF ? lo - (int32_t)RING_HALF : 0;
-#pragma unroll
-        for (uint32_t i = 0; i < RING_PEND; ++i) {
-          const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
-          pend[i] = w >= 0 ? words[w] : 0u;
-        }
-        __syncthreads();
+      for (uint32_t i = 0; i < RING_PEND; ++i) {
+        const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
+        ring[(w < 0 ? 0 : w) & (RING_WORDS - 1)] = w < 0 ? ring[0] : pend[i];
       }
-      pw = wi;
-      curw = __builtin_amdgcn_readfirstlane(ring[wi & (RING_WORDS - 1)]);
+      lo = lo > (int32_t)RING_HALF ? lo - (int32_t)RING_HALF : 0;
+#pragma unroll
+      for (uint32_t i = 0; i < RING_PEND; ++i) {
+        const int32_t w = lo - (int32_t)RING_HALF + (int32_t)(lane + 64 * i);
+        pend[i] = words[w < 0 ? 0 : w];
+      }
+      __syncthreads();
     }
-    return (curw >> ((pos & 3u) * 8u)) & 0xFFu;
+    cw = __builtin_amdgcn_readfirstlane(ring[wnext & (RING_WORDS - 1)]);
+    --wnext;
+    nbb = 4;
   };
-
+  fetch_word();
+  {
+    const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of the last word beyond E
+    cw <<= 8u * skip;
+    nbb -= skip;
+  }
+  auto next_byte = [&]() -> uint32_t {
+    if (nbb == 0) fetch_word();
+    const uint32_t r = cw >> 24;
+    cw <<= 8;
+    --nbb;
+    --pos;
+    return r;
+  };
   uint32_t state = next_byte() << 24;
   state |= next_byte() << 16;
   state |= next_byte() << 8;
   state |= next_byte();
 
-  uint32_t outv = 0, nw = 0;
-  for (uint32_t g = 0; g < nsym; ++g) {
+  // symbol step. With a decode table (x -> symbol | freq | x - cum) the
+  // chain stays on the scalar unit (one scalar-cache load per symbol);
+  // otherwise the symbol is the ballot count of per-lane CDF compares.
+  // the table is read through the constant address space so that the
+  // uniform-index loads go through the scalar cache (s_load)
+  typedef __attribute__((address_space(4))) const uint32_t ctab_t;
+  ctab_t* tab = (ctab_t*)a.table;
+  auto decode_one = [&]() -> uint32_t {
     const uint32_t x = state & (FCS_RANS_M - 1);
-    const uint32_t s = (uint32_t)__popcll(__ballot(x >= my_cum)) - 1u;
-    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)s);
-    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_cum, (int)s);
-    state = f * (state >> FCS_RANS_LOGM) + x - c;
-    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | next_byte();
-    nw |= s << ((g & 7u) * 4u);
-    if ((g & 7u) == 7u) {
-      const uint32_t wl = (g >> 3) & 63u;
-      outv = lane == wl ? nw : outv;  // word wl of the 64-word group lives in lane wl
-      nw = 0;
-      if (wl == 63u) nibw[(g >> 9) * 64u + lane] = outv;
+    uint32_t sym;
+    if (tab) {
+      const uint32_t e = tab[x];
+      sym = e >> 25;
+      state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (e & 0xFFFu);
+    } else {
+      sym = (uint32_t)__popcll(__ballot(x >= my_next));
+      const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)sym);
+      const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_c, (int)sym);
+      state = f * (state >> FCS_RANS_LOGM) + x - c;
     }
+    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | next_byte();
+    return sym;
+  };
+
+  // 8 symbols -> one nibble word; word k of a 64-word group lives in lane k
+  uint32_t outv = 0;
+  const uint32_t nfull = nsym >> 3;
+  for (uint32_t wd = 0; wd < nfull; ++wd) {
+    uint32_t word = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) word |= decode_one() << (4u * k);
+    outv = lane == (wd & 63u) ? word : outv;
+    if ((wd & 63u) == 63u) nibw[(wd >> 6) * 64u + lane] = outv;
+  }
+  if (nsym & 7u) {
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < (nsym & 7u); ++k) word |= decode_one() << (4u * k);
+    outv = lane == (nfull & 63u) ? word : outv;
   }
   const uint32_t total_words = (nsym + 7) >> 3;
-  if (nsym & 7u) outv = lane == ((nsym >> 3) & 63u) ? nw : outv;
   const uint32_t stored = (nsym >> 9) * 64u;
   if (lane < total_words - stored) nibw[stored + lane] = outv;
 
   uint32_t status = 0;
   if (nsym & 1u) {
-    const uint32_t x = state & (FCS_RANS_M - 1);
-    const uint32_t s = (uint32_t)__popcll(__ballot(x >= my_cum)) - 1u;
-    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)s);
-    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_cum, (int)s);
-    state = f * (state >> FCS_RANS_LOGM) + x - c;
-    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | next_byte();
-    if (s != 0) status |= FCS_ST_RANS_STATE;
+    if (decode_one() != 0) status |= FCS_ST_RANS_STATE;
   }
   if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
   if (lane == 0) a.res[b] = RansResult{status, 0u};
@@ -275,8 +303,10 @@ __global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
 
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s) {
   if (a.nlist == 0) return hipSuccess;
-  if (mode == FCS_RANS_WAVE) {
-    hipLaunchKernelGGL(rans_wave_kernel, dim3(a.nlist), dim3(64), 0, s, a);
+  if (mode == FCS_RANS_WAVE || mode == FCS_RANS_WAVE_BALLOT) {
+    RansArgs w = a;
+    if (mode == FCS_RANS_WAVE_BALLOT) w.table = nullptr;
+    hipLaunchKernelGGL(rans_wave_kernel, dim3(a.nlist), dim3(64), 0, s, w);
   } else {
     hipLaunchKernelGGL(rans_lane_kernel, dim3((a.nlist + 255) / 256), dim3(256), 0, s, a);
   }

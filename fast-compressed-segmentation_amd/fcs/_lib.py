@@ -101,10 +101,12 @@ class TimingStruct(ctypes.Structure):
         ("rans_bytes", ctypes.c_uint64),
         ("tune_lane_ms", ctypes.c_float),
         ("tune_wave_ms", ctypes.c_float),
+        ("tune_ballot_ms", ctypes.c_float),
+        ("pad", ctypes.c_float),
     ]
 
 
-RANS_AUTO, RANS_LANE, RANS_WAVE = 0, 1, 2
+RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT = 0, 1, 2, 3
 OPT_RANS_MAPPING = 1
 
 
