@@ -28,10 +28,8 @@ struct DecodeReq {
 #define FCS_REQ_PYRAMID 0x100u
 
 // Ops kernel launch geometry and LDS budget.
-#define FCS_OPS_THREADS 1024
+#define FCS_OPS_THREADS 256
 #define FCS_OPS_WAVES (FCS_OPS_THREADS / 64)
-// Largest level held in LDS as 16-bit palette indices (32^3 nodes).
-#define FCS_LDS_LEVEL_MAX 32768u
 // 16-bit node value: bit 15 set = reference to another node of the same level.
 #define FCS_REF16 0x8000u
 #define FCS_REF32 0x80000000u

@@ -171,16 +171,15 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
   if (int rc = ensure_req(v, reqs.size())) return rc;
   HIP_TRY(c, hipMemcpyAsync(v->d_req, reqs.data(), reqs.size() * sizeof(DecodeReq), hipMemcpyHostToDevice,
                             c->stream));
-  const uint32_t grid = (uint32_t)std::min<size_t>(reqs.size(), (size_t)c->num_cu * 2);
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
   if (v->label_bits == 32) {
     OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, v->d_req,
                         (uint32_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, grid, c->stream));
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, c->stream));
   } else {
     OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, v->d_req,
                         (uint64_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, grid, c->stream));
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, c->stream));
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
   HIP_TRY(c, hipMemcpyAsync(status.data(), v->d_req_status, reqs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,

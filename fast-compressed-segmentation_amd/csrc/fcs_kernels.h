@@ -73,8 +73,9 @@ struct MarchArgs {
 
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s);
 size_t fcs_ops_lds_bytes(uint32_t L);
+// Persistent launch: min(nreq, resident workgroups per CU x num_cu) workgroups.
 template <typename LT>
-hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t grid, hipStream_t s);
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr);
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s);
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s);

@@ -1,5 +1,28 @@
 // Op decoder for gfx950: decodes (brick, LOD) requests from the device
-// nibble streams into the brick cache. See the comment block below.
+// nibble streams into the brick cache, bit-exactly like decode.wgsl:344-508 /
+// brickDecoding.hpp:174-242 (with the root stop bit honoured).
+//
+// Decoding a brick at LOD t = computing the levels 1..t of its pyramid.
+// Level l+1 depends only on level l and on *smaller* nodes of level l+1, so
+// each level is one parallel step of a workgroup, persistent over a queue of
+// requests. Per level, three phases separated by barriers:
+//   P1  children of stopped parents copy the parent (one 16-byte LDS store
+//       per parent); the level's nibble span is parsed 8 nibbles per lane: a
+//       nibble is a Pd delta iff the run of 12/13-nibbles before it (back to
+//       the level start) has odd length. Each wave publishes its op / palette
+//       counts for both possible entering parities plus its exit parity, so
+//       one barrier per 2048-nibble tile resolves every lane's op index and
+//       palette cursor. Each op becomes a palette index (Pa/P0/Pd), the
+//       parent's value (Rp) or a reference to a smaller node (Rx/Ry/Rz).
+//   P2  references are chased; active nodes of the next level are counted.
+//   P3  the next level's active list is compacted; labels are written.
+// Intermediate levels live in LDS as 16-bit palette indices (bit 15 = node
+// reference). The requested (final) level is never materialised in LDS: op
+// results are kept per op (compact, in LDS) or per node in the cache region
+// (when there are too many ops), and the labels are written per parent as
+// 8 consecutive voxels (two or four 16-byte stores). Children of stopped
+// parents are written straight from the parent's value. A level whose nodes
+// are all stopped ends the brick: the rest is a direct expansion.
 #include <hip/hip_runtime.h>
 
 #include "fcs_common.h"
@@ -8,23 +31,62 @@
 
 namespace {
 
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+template <uint32_t NT_, uint32_t VCAP_, uint32_t BIG_, uint32_t ACT_, uint32_t SW_, uint32_t KCAP_, uint32_t PCAP_>
+struct OpsCfg {
+  static constexpr uint32_t NT = NT_;      // threads per workgroup
+  static constexpr uint32_t NW = NT_ / 64;  // waves
+  static constexpr uint32_t VCAP = VCAP_;  // two ping-pong level buffers (u16)
+  static constexpr uint32_t BIG = BIG_;    // level buffer beyond VCAP nodes (u16)
+  static constexpr uint32_t ACT = ACT_;    // active-parent list (u16)
+  static constexpr uint32_t SW = SW_;      // stop-bit words per level
+  static constexpr uint32_t KCAP = KCAP_;  // compact final-level op results (u16)
+  static constexpr uint32_t PCAP = PCAP_;  // palette entries staged in LDS
+};
+// bricks up to 32^3: four waves, ~46 KB of LDS (three workgroups per CU)
+using CfgSmall = OpsCfg<256, 4096, 0, 4096, 128, 8192, 1024>;
+// 64^3 bricks: sixteen waves, the 32768-node level in LDS (~155 KB)
+using CfgLarge = OpsCfg<1024, 4096, 32768, 32768, 1024, 0, 0>;
 
-// ============================================================ op decoding
-// One workgroup (16 waves) per request, persistent over a request queue.
-// Level l -> l+1 for the 8^(l+1) child nodes, all in LDS as 16-bit palette
-// indices (bit 15 = reference to an earlier node of the same level):
-//   1. active parents (no stop bit) compacted in Morton order  -> ACT
-//   2. children of stopped parents copy the parent; stop bits init
-//   3. the level's nibble span is parsed 8 nibbles per lane: a nibble is a
-//      delta iff the run of 12/13-nibbles before it (back to the level start)
-//      has odd length; op index / palette cursor by block prefix scan
-//   4. each op resolves to a palette index or a node reference
-//   5. references are chased (always to a smaller node index)
-//   6. labels (palette[idx]) stored coalesced to the brick cache
-// This restates decode.wgsl:404-508 / brickDecoding.hpp:174-242 level by
-// level: the value of node n at level l+1 only depends on level l and on
-// nodes < n of level l+1, so each level is one parallel step.
+constexpr uint32_t SCR_WORDS = 128;
+constexpr uint32_t SCR_WAVESUM = 0;   // NW words: active-node counts per wave
+constexpr uint32_t SCR_REC = 16;      // 2 x 48 words: parse records per wave
+constexpr uint32_t SCR_STATUS = 120;
+constexpr uint32_t SCR_NS = 122;      // nibble where the next level starts
+constexpr uint32_t SCR_IP = 123;      // palette cursor at the next level's start
+constexpr uint32_t SCR_REQ = 124;
+
+template <class C, typename LT>
+struct Layout {
+  static constexpr size_t sA = 0;
+  static constexpr size_t sB = sA + C::VCAP * 2;
+  static constexpr size_t big = sB + C::VCAP * 2;
+  static constexpr size_t act = big + C::BIG * 2;
+  static constexpr size_t wpre = act + C::ACT * 2;
+  static constexpr size_t opv = (wpre + C::SW * 2 + 15) / 16 * 16;
+  static constexpr size_t st0 = (opv + C::KCAP * 2 + 15) / 16 * 16;
+  static constexpr size_t st1 = st0 + C::SW * 4;
+  static constexpr size_t scr = st1 + C::SW * 4;
+  static constexpr size_t spal = (scr + SCR_WORDS * 4 + 15) / 16 * 16;
+  static constexpr size_t bytes = spal + C::PCAP * sizeof(LT);
+};
+
+template <typename LT>
+struct Lds {
+  uint16_t* sA;
+  uint16_t* sB;
+  uint16_t* big;
+  uint16_t* act;   // active parents of the level being expanded, Morton order
+  uint16_t* wpre;  // active parents before each stop word (rank lookup)
+  uint16_t* opv;   // compact final-level op results
+  uint32_t* st0;
+  uint32_t* st1;
+  uint32_t* scr;
+  LT* spal;
+};
+
+enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2 };
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
@@ -37,44 +99,136 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   return v;
 }
 
-// exclusive block scan over FCS_OPS_THREADS lanes; sh needs FCS_OPS_WAVES+1 words
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
-  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-  const uint32_t inc = wave_incl_scan(v);
-  if (lane == 63) sh[wave] = inc;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const uint32_t w = threadIdx.x < FCS_OPS_WAVES ? sh[threadIdx.x] : 0u;
-    const uint32_t wi = wave_incl_scan(w);
-    if (threadIdx.x < FCS_OPS_WAVES) sh[threadIdx.x] = wi - w;
-    if (threadIdx.x == FCS_OPS_WAVES - 1) sh[FCS_OPS_WAVES] = wi;
+__device__ __forceinline__ bool is_stopped(const uint32_t* st, uint32_t m) { return (st[m >> 5] >> (m & 31)) & 1u; }
+
+// index of active node m among the active nodes of its level
+__device__ __forceinline__ uint32_t rank_of(const uint32_t* st, const uint16_t* wpre, uint32_t m) {
+  return wpre[m >> 5] + __popc(~st[m >> 5] & ((1u << (m & 31)) - 1u));
+}
+
+template <typename LT>
+__device__ __forceinline__ uint4 splat16(LT v) {
+  if constexpr (sizeof(LT) == 4) {
+    const uint32_t u = (uint32_t)v;
+    return make_uint4(u, u, u, u);
+  } else {
+    const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    return make_uint4(lo, hi, lo, hi);
   }
-  __syncthreads();
-  const uint32_t r = sh[wave] + inc - v;
-  total = sh[FCS_OPS_WAVES];
-  __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ uint32_t nib_word(const uint32_t* nibw, uint32_t wi) { return nibw[wi]; }
-
-struct OpsShared {
-  uint16_t* bigv;   // FCS_LDS_LEVEL_MAX
-  uint16_t* smv;    // 4096
-  uint16_t* act;    // act_cap
-  uint32_t* sa;     // stop bits A (1024 words)
-  uint32_t* sb;     // stop bits B (1024 words)
-  uint32_t* scr;    // scan scratch (64 words)
-};
-
-template <typename LT>
-__device__ void fill_const(LT* o, uint64_t count, LT v) {
-  for (uint64_t i = threadIdx.x; i < count; i += FCS_OPS_THREADS) o[i] = v;
 }
 
 template <typename LT>
-__device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_t r) {
+__device__ __forceinline__ void store8(LT* p, const LT (&v)[8], bool al) {
+  if (al) {
+    uint4* q = reinterpret_cast<uint4*>(p);
+    if constexpr (sizeof(LT) == 4) {
+      q[0] = make_uint4(v[0], v[1], v[2], v[3]);
+      q[1] = make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        q[i] = make_uint4((uint32_t)v[2 * i], (uint32_t)(v[2 * i] >> 32), (uint32_t)v[2 * i + 1],
+                          (uint32_t)(v[2 * i + 1] >> 32));
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] = v[i];
+  }
+}
+
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// store `count` labels of value v
+template <typename LT, uint32_t NT>
+__device__ __forceinline__ void fill_const(LT* o, uint64_t count, LT v) {
   const uint32_t tid = threadIdx.x;
+  constexpr uint32_t PER = 16 / sizeof(LT);
+  if (aligned16(o)) {
+    const uint4 q = splat16(v);
+    uint4* o4 = reinterpret_cast<uint4*>(o);
+    const uint64_t n4 = count / PER;
+    for (uint64_t i = tid; i < n4; i += NT) o4[i] = q;
+    for (uint64_t i = n4 * PER + tid; i < count; i += NT) o[i] = v;
+  } else {
+    for (uint64_t i = tid; i < count; i += NT) o[i] = v;
+  }
+}
+
+// level lt labels when every node of level lv is stopped: each node copies
+// its level-lv ancestor
+template <typename LT, uint32_t NT>
+__device__ __forceinline__ void expand_level(LT* o, uint32_t lt, uint32_t lv, const uint16_t* vl, const LT* pal,
+                                             uint32_t P) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t shift = 3 * (lt - lv);  // >= 3: 8 consecutive voxels share an ancestor
+  const uint32_t n = 1u << (3 * lt);
+  if (aligned16(o)) {
+    constexpr uint32_t PER = 16 / sizeof(LT);
+    uint4* o4 = reinterpret_cast<uint4*>(o);
+    for (uint32_t i = tid; i < n / PER; i += NT) {
+      const uint32_t v = vl[(i * PER) >> shift];
+      o4[i] = splat16(pal[v < P ? v : P - 1]);
+    }
+  } else {
+    for (uint32_t i = tid; i < n; i += NT) {
+      const uint32_t v = vl[i >> shift];
+      o[i] = pal[v < P ? v : P - 1];
+    }
+  }
+}
+
+// final level from compact op results: 8 labels per parent
+template <typename LT, uint32_t NT>
+__device__ __forceinline__ void out_compact(LT* olev, uint32_t N, const uint32_t* scur, const uint16_t* wpre,
+                                            const uint16_t* vcur, const uint16_t* opv, const LT* pl, uint32_t P) {
+  const bool al = aligned16(olev);
+  for (uint32_t m = threadIdx.x; m < N; m += NT) {
+    LT v8[8];
+    if (is_stopped(scur, m)) {
+      const uint32_t v = vcur[m];
+      const LT u = pl[v < P ? v : P - 1];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) v8[c] = u;
+    } else {
+      const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * rank_of(scur, wpre, m));
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t v = (w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        v8[c] = pl[v < P ? v : P - 1];
+      }
+    }
+    store8(olev + 8 * m, v8, al);
+  }
+}
+
+// final level from per-node op results in the cache region
+template <typename LT, uint32_t NT>
+__device__ __forceinline__ void out_global(LT* olev, uint32_t N, const uint32_t* scur, const uint16_t* vcur,
+                                           const LT* pl, uint32_t P) {
+  const bool al = aligned16(olev);
+  for (uint32_t m = threadIdx.x; m < N; m += NT) {
+    LT v8[8];
+    if (is_stopped(scur, m)) {
+      const uint32_t v = vcur[m];
+      const LT u = pl[v < P ? v : P - 1];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) v8[c] = u;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t v = (uint32_t)olev[8 * m + c];
+        v8[c] = pl[v < P ? v : P - 1];
+      }
+    }
+    store8(olev + 8 * m, v8, al);
+  }
+}
+
+template <typename LT, class C>
+__device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t r) {
+  constexpr uint32_t NT = C::NT, NW = C::NW;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const DecodeReq q = a.reqs[r];
   const uint32_t b = q.brick, t = q.lod & 0xFFu;
   const bool pyr = (q.lod & FCS_REQ_PYRAMID) != 0;
@@ -99,134 +253,104 @@ __device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_
   const LT p0 = pal[0];
   if ((root & 1u) || t == 0 || (status & FCS_ST_STREAM_OVERRUN)) {
     // constant brick (root stop bit) or LOD 0: every voxel = palette[0]
-    fill_const(o, pyr ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t)), p0);
+    fill_const<LT, NT>(o, pyr ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t)), p0);
     if (tid == 0) a.status[r] = status;
     return;
   }
+  const bool staged = C::PCAP != 0 && P <= C::PCAP;
+  if (staged)
+    for (uint32_t i = tid; i < P; i += NT) S.spal[i] = pal[i];
+  const LT* lp = staged ? static_cast<const LT*>(S.spal) : pal;
   if (pyr && tid == 0) o[0] = p0;
-  uint16_t* vcur = S.smv;
-  uint32_t* scur = S.sa;
-  uint32_t* snext = S.sb;
-  if (tid == 0) { vcur[0] = 0; scur[0] = 0; S.scr[60] = 0; }
+  if (tid == 0) {
+    // level 0: the root (palette index 0, active); level 1 not stopped yet
+    S.sA[0] = 0;
+    S.act[0] = 0;
+    S.wpre[0] = 0;
+    S.st0[0] = 0;
+    S.st1[0] = 0;
+    S.scr[SCR_STATUS] = 0;
+    S.scr[SCR_NS] = 1;
+    S.scr[SCR_IP] = 0;
+  }
   __syncthreads();
+
+  uint16_t* vcur = S.sA;
+  uint32_t* scur = S.st0;
+  uint32_t* snext = S.st1;
   // level l's ops start at nibble ns with palette cursor ip0 (nibble 0 is the
   // root op); each level's end is found while parsing it
-  uint32_t ns = 1, ip0 = 0;
-  const uint32_t w_stream_end = (nsym + 7) >> 3;
+  uint32_t A = 1, ns = 1, ip0 = 0, tile = 0;
+  const uint32_t w_end = (nsym + 7) >> 3;
+  const uint32_t ne = nsym;  // parse bound; a level ends after its K-th op
 
   for (uint32_t l = 0; l < t; ++l) {
-    const uint32_t N = 1u << (3 * l), C = N << 3;
+    const uint32_t N = 1u << (3 * l), Cn = N << 3, K = 8 * A;
     const bool last = (l + 1 == t);
-    const bool gnext = last && C > FCS_LDS_LEVEL_MAX;  // final level lives in the cache region
+    const uint32_t mode = !last ? MODE_LEVEL : ((C::KCAP != 0 && K <= C::KCAP) ? MODE_COMPACT : MODE_GLOBAL);
     LT* olev = o + (pyr ? fcs_pyramid_level_offset(l + 1) : 0ull);
-    uint16_t* vnext = S.bigv;
+    uint16_t* vnext = (Cn <= C::VCAP) ? (vcur == S.sA ? S.sB : S.sA) : S.big;
 
-    // 1. active-parent compaction (32 parents per stop word)
-    const uint32_t nw = (N + 31) >> 5;
-    uint32_t abits = 0;
-    if (tid < nw) {
-      const uint32_t valid = (N >= 32) ? 0xFFFFFFFFu : ((1u << N) - 1u);
-      abits = ~scur[tid] & valid;
+    // ---- P1: stopped children; parse and resolve the level's ops
+    if (mode == MODE_LEVEL) {
+      for (uint32_t m = tid; m < N; m += NT)
+        if (is_stopped(scur, m)) {
+          const uint32_t v = vcur[m] | ((uint32_t)vcur[m] << 16);
+          *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(v, v, v, v);
+        }
     }
-    uint32_t A;
-    const uint32_t abase = block_excl_scan(__popc(abits), S.scr, A);
-    {
-      uint32_t k = abase, bits = abits;
-      while (bits) {
-        const uint32_t j = __ffs(bits) - 1;
-        bits &= bits - 1;
-        S.act[k++] = (uint16_t)(tid * 32 + j);
-      }
-    }
-    // 2. children of stopped parents; stop bits of the next level
-    for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) {
-      const uint32_t m = n >> 3;
-      if ((scur[m >> 5] >> (m & 31)) & 1u) {
-        const uint16_t v = vcur[m];
-        if (gnext) olev[n] = (LT)v; else vnext[n] = v;
-      }
-    }
-    if (!last) {
-      for (uint32_t w = tid; w < ((C + 31) >> 5); w += FCS_OPS_THREADS) {
-        const uint32_t pb = (scur[w >> 3] >> ((w & 7) * 4)) & 0xFu;  // 4 parents
-        uint32_t m = 0;
-        m |= (pb & 1u) ? 0x000000FFu : 0u;
-        m |= (pb & 2u) ? 0x0000FF00u : 0u;
-        m |= (pb & 4u) ? 0x00FF0000u : 0u;
-        m |= (pb & 8u) ? 0xFF000000u : 0u;
-        snext[w] = m;
-      }
-    }
-    if (tid == 0) { S.scr[61] = 0; S.scr[62] = ns; S.scr[63] = ip0; }
-    __syncthreads();
-
-    // 3-4. parse this level's ops and resolve them to values / references
-    const uint32_t K = 8 * A;
     uint32_t op_base = 0, pa_base = 0, carry = 0;  // carry: next nibble is a delta
-    const uint32_t w_first = ns >> 3, w_end = w_stream_end;
-    const uint32_t ne = nsym;  // parse bound; the level ends after its K-th op
     const uint32_t side = 1u << (l + 1);
-    for (uint32_t wt = w_first; wt < w_end && op_base < K; wt += FCS_OPS_THREADS) {
+    for (uint32_t wt = ns >> 3; wt < w_end && op_base < K; wt += NT, ++tile) {
       const uint32_t wi = wt + tid;
       const bool inw = wi < w_end;
-      const uint32_t word = inw ? nib_word(nibw, wi) : 0u;
+      const uint32_t word = inw ? nibw[wi] : 0u;
       const uint32_t g0 = wi * 8;
-      // trailing run of 12/13 nibbles over the valid nibbles of this word
-      uint32_t tr = 0;
-      for (uint32_t j = 0; j < 8; ++j) {
-        const uint32_t g = g0 + j;
-        const uint32_t v = (word >> (4 * j)) & 0xFu;
-        const bool is12 = inw && g >= ns && g < ne && (v >> 1) == FCS_OP_PD;
-        tr = is12 ? tr + 1 : 0;
-      }
-      const bool full = (tr == 8);
-      const uint64_t Pm = __ballot(tr & 1u);
-      const uint64_t Nm = __ballot(!full);
-      const uint32_t lane = lane_id(), wave = tid >> 6;
-      // wave exit parity record
-      if (lane == 0) {
-        const uint32_t has = Nm != 0;
-        const uint32_t hi = has ? 63 - __clzll(Nm) : 0;
-        S.scr[32 + wave] = has ? (2u | ((uint32_t)(Pm >> hi) & 1u)) : 0u;
-      }
-      __syncthreads();
-      uint32_t par;  // parity entering this lane
+      // trailing run of 12/13 nibbles over this lane's valid nibbles, and the
+      // op / palette-advance counts for both entering parities
+      uint32_t tr = 0, cnt0 = 0, cnt1 = 0;
       {
-        const uint64_t below = Nm & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        if (below) {
-          par = (uint32_t)(Pm >> (63 - __clzll(below))) & 1u;
-        } else {
-          par = carry;
-          for (int w = (int)wave - 1; w >= 0; --w) {
-            const uint32_t rec = S.scr[32 + w];
-            if (rec & 2u) { par = rec & 1u; break; }
+        uint32_t pd0 = 0, pd1 = 1;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+          const uint32_t g = g0 + j;
+          const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
+          const bool valid = inw && g >= ns && g < ne;
+          tr = (valid && op == FCS_OP_PD) ? tr + 1 : 0;
+          if (g == ns) { pd0 = 0; pd1 = 0; }
+          const uint32_t inc = 1u | ((op == FCS_OP_PA) ? 0x10000u : 0u);
+          if (valid) {
+            if (pd0) pd0 = 0; else { cnt0 += inc; pd0 = (op == FCS_OP_PD); }
+            if (pd1) pd1 = 0; else { cnt1 += inc; pd1 = (op == FCS_OP_PD); }
           }
         }
       }
-      uint32_t tile_exit = carry;
-      for (int w = FCS_OPS_WAVES - 1; w >= 0; --w) {
-        const uint32_t rec = S.scr[32 + w];
-        if (rec & 2u) { tile_exit = rec & 1u; break; }
+      const uint64_t Pm = __ballot(tr & 1u);
+      const uint64_t Nm = __ballot(tr != 8);
+      const uint64_t below = Nm & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+      const bool fixed = below != 0;  // entering parity set by a lane of this wave
+      const uint32_t pfix = fixed ? (uint32_t)(Pm >> (63 - __clzll(below))) & 1u : 0u;
+      const uint32_t cfix = pfix ? cnt1 : cnt0;
+      const uint32_t v0 = fixed ? cfix : cnt0, v1 = fixed ? cfix : cnt1;
+      const uint32_t i0 = wave_incl_scan(v0), i1 = wave_incl_scan(v1);
+      uint32_t* R = S.scr + SCR_REC + (tile & 1u) * 48u;
+      if (lane == 63) {
+        R[3 * wave] = Nm ? (2u | ((uint32_t)(Pm >> (63 - __clzll(Nm))) & 1u)) : 0u;
+        R[3 * wave + 1] = i0;
+        R[3 * wave + 2] = i1;
       }
-      // count ops / palette advances of this lane
-      uint32_t nops = 0, npa = 0;
-      {
-        uint32_t pd = par;
-        for (uint32_t j = 0; j < 8; ++j) {
-          const uint32_t g = g0 + j;
-          if (!inw || g >= ne) break;
-          if (g < ns) continue;
-          if (g == ns) pd = 0;
-          const uint32_t v = (word >> (4 * j)) & 0xFu;
-          if (pd) { pd = 0; continue; }
-          ++nops;
-          npa += ((v >> 1) == FCS_OP_PA);
-          pd = ((v >> 1) == FCS_OP_PD);
-        }
+      __syncthreads();
+      uint32_t X = carry, base = 0, myX = 0, mybase = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < NW; ++w) {
+        if (w == wave) { myX = X; mybase = base; }
+        const uint32_t rec = R[3 * w];
+        base += X ? R[3 * w + 2] : R[3 * w + 1];
+        X = (rec & 2u) ? (rec & 1u) : X;
       }
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan(nops | (npa << 16), S.scr, tot);
-      // process ops of this lane
+      const uint32_t par = fixed ? pfix : myX;
+      const uint32_t ex = mybase + (myX ? i1 - v1 : i0 - v0);
+      // resolve this lane's ops
       {
         uint32_t k = op_base + (ex & 0xFFFFu);
         uint32_t ipx = ip0 + pa_base + (ex >> 16);
@@ -240,7 +364,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_
           if (pd) { pd = 0; continue; }
           const uint32_t op = v >> 1, stop = v & 1u;
           pd = (op == FCS_OP_PD);
-          if (k >= K) break;  // the rest of the word belongs to the next level
+          if (k >= K) break;  // the rest belongs to the next level
           const uint32_t m = S.act[k >> 3], c = k & 7u, n = 8 * m + c;
           uint32_t val;  // palette index, or FCS_REF32 | node
           switch (op) {
@@ -271,7 +395,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_
               break;
             case FCS_OP_PD: {
               if (g + 1 >= nsym) status |= FCS_ST_STREAM_OVERRUN;
-              const uint32_t dn = (j < 7) ? ((word >> (4 * (j + 1))) & 0xFu) : (nib_word(nibw, wi + 1) & 0xFu);
+              const uint32_t dn = (j < 7) ? ((word >> (4 * (j + 1))) & 0xFu) : (nibw[wi + 1] & 0xFu);
               const int64_t idx = (int64_t)ipx - (int64_t)dn - 1;
               if (idx < 0 || idx >= (int64_t)P) { status |= FCS_ST_PALETTE_RANGE; val = P - 1; }
               else val = (uint32_t)idx;
@@ -281,50 +405,41 @@ __device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_
               status |= FCS_ST_BAD_OP;
               val = 0;
           }
-          if (gnext) {
-            olev[n] = (LT)val;
+          const uint32_t v16 = (val & FCS_REF32) ? (FCS_REF16 | (val & 0x7FFFu)) : val;
+          if (mode == MODE_LEVEL) {
+            vnext[n] = (uint16_t)v16;
+            if (stop) atomicOr(&snext[n >> 5], 1u << (n & 31));
+          } else if (mode == MODE_COMPACT) {
+            S.opv[k] = (uint16_t)v16;
           } else {
-            vnext[n] = (val & FCS_REF32) ? (uint16_t)(FCS_REF16 | (val & 0x7FFFu)) : (uint16_t)val;
-            if (stop && !last) atomicOr(&snext[n >> 5], 1u << (n & 31));
+            olev[n] = (LT)val;
           }
           if (k == K - 1) {
             // last op of the level: the next level starts after it (and its delta)
-            S.scr[62] = g + 1 + (op == FCS_OP_PD ? 1u : 0u);
-            S.scr[63] = ipx;
+            S.scr[SCR_NS] = g + 1 + (op == FCS_OP_PD ? 1u : 0u);
+            S.scr[SCR_IP] = ipx;
           }
           ++k;
         }
       }
-      op_base += tot & 0xFFFFu;
-      pa_base += tot >> 16;
-      carry = tile_exit;
-      __syncthreads();
+      op_base += base & 0xFFFFu;
+      pa_base += base >> 16;
+      carry = X;
     }
     if (op_base < K) status |= FCS_ST_STREAM_OVERRUN | FCS_ST_NSYM_MISMATCH;
     __syncthreads();
-    ns = S.scr[62];
-    ip0 = S.scr[63];
+    ns = S.scr[SCR_NS];
+    ip0 = S.scr[SCR_IP];
     if (l + 1 == L && ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
 
-    // 5. chase references (targets are always smaller node indices)
-    // A valid reference always points to a smaller node, so every chain
-    // strictly decreases; anything else is corrupt input and stops the chase.
-    if (gnext) {
-      for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) {
-        uint32_t v = (uint32_t)olev[n];
-        if (v & FCS_REF32) {
-          uint32_t cur = n;
-          do {
-            const uint32_t nx = v & ~FCS_REF32;
-            if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-            cur = nx;
-            v = (uint32_t)olev[nx];
-          } while (v & FCS_REF32);
-          olev[n] = (LT)v;
-        }
-      }
-    } else {
-      for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) {
+    // ---- P2: chase references. A valid reference points to a smaller node,
+    // so a chain strictly decreases; anything else is corrupt input. Only op
+    // children can hold references.
+    uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
+    const uint32_t nw = (Cn + 31) >> 5;
+    if (mode == MODE_LEVEL) {
+      for (uint32_t k = tid; k < K; k += NT) {
+        const uint32_t n = 8 * S.act[k >> 3] + (k & 7u);
         uint32_t v = vnext[n];
         if (v & FCS_REF16) {
           uint32_t cur = n;
@@ -337,58 +452,144 @@ __device__ void decode_request(const OpsArgs<LT>& a, const OpsShared& S, uint32_
           vnext[n] = (uint16_t)v;
         }
       }
+      // active nodes of level l+1, over consecutive stop words per thread
+      const uint32_t wpt = (nw + NT - 1) / NT;
+      w0 = min(nw, tid * wpt);
+      w1 = min(nw, w0 + wpt);
+      const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
+      for (uint32_t w = w0; w < w1; ++w) cnt += __popc(~snext[w] & valid);
+      inc = wave_incl_scan(cnt);
+      if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
+    } else if (mode == MODE_COMPACT) {
+      for (uint32_t k = tid; k < K; k += NT) {
+        uint32_t v = S.opv[k];
+        if (v & FCS_REF16) {
+          uint32_t cur = 8 * S.act[k >> 3] + (k & 7u);
+          for (;;) {
+            const uint32_t nb = v & 0x7FFFu;
+            if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
+            const uint32_t p = nb >> 3;
+            if (is_stopped(scur, p)) { v = vcur[p]; break; }
+            v = S.opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
+            if (!(v & FCS_REF16)) break;
+            cur = nb;
+          }
+          S.opv[k] = (uint16_t)v;
+        }
+      }
+    } else {
+      for (uint32_t k = tid; k < K; k += NT) {
+        const uint32_t n = 8 * S.act[k >> 3] + (k & 7u);
+        uint32_t v = (uint32_t)olev[n];
+        if (v & FCS_REF32) {
+          uint32_t cur = n;
+          for (;;) {
+            const uint32_t nb = v & ~FCS_REF32;
+            if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
+            const uint32_t p = nb >> 3;
+            if (is_stopped(scur, p)) { v = vcur[p]; break; }
+            v = (uint32_t)olev[nb];
+            if (!(v & FCS_REF32)) break;
+            cur = nb;
+          }
+          olev[n] = (LT)v;
+        }
+      }
     }
     __syncthreads();
 
-    // 6. labels out
-    if (gnext) {
-      for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) {
-        uint32_t v = (uint32_t)olev[n];
-        olev[n] = pal[v < P ? v : P - 1];
+    // ---- P3: next level's active list; labels out
+    bool ended = false;
+    if (mode == MODE_LEVEL) {
+      uint32_t An = 0, wb = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < NW; ++w) {
+        const uint32_t s = S.scr[SCR_WAVESUM + w];
+        wb += (w < wave) ? s : 0u;
+        An += s;
       }
-    } else if (last || pyr) {
-      for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) {
-        const uint32_t v = vnext[n];
-        olev[n] = pal[v < P ? v : P - 1];
-      }
-    }
-    if (!last) {
-      // next level's parents
-      if (C <= 4096) {
-        for (uint32_t n = tid; n < C; n += FCS_OPS_THREADS) S.smv[n] = vnext[n];
-        vcur = S.smv;
+      if (pyr)
+        for (uint32_t n = tid; n < Cn; n += NT) {
+          const uint32_t v = vnext[n];
+          olev[n] = lp[v < P ? v : P - 1];
+        }
+      if (An == 0) {
+        // every node of level l+1 is stopped: the remaining levels are copies
+        for (uint32_t lt = pyr ? l + 2 : t; lt <= t; ++lt) {
+          LT* ol = o + (pyr ? fcs_pyramid_level_offset(lt) : 0ull);
+          if (staged) expand_level<LT, NT>(ol, lt, l + 1, vnext, S.spal, P);
+          else expand_level<LT, NT>(ol, lt, l + 1, vnext, pal, P);
+        }
+        if (t == L && ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
+        ended = true;
       } else {
-        vcur = S.bigv;  // only feeds a final level held in the cache region
+        uint32_t k0 = wb + inc - cnt;
+        const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
+        for (uint32_t w = w0; w < w1; ++w) {
+          S.wpre[w] = (uint16_t)k0;
+          uint32_t bits = ~snext[w] & valid;
+          while (bits) {
+            const uint32_t j = __ffs(bits) - 1;
+            bits &= bits - 1;
+            S.act[k0++] = (uint16_t)(w * 32 + j);
+          }
+        }
+        if (l + 2 < t) {
+          // stop bits of level l+2: children of stopped nodes are stopped
+          for (uint32_t w = tid; w < Cn / 4; w += NT) {
+            const uint32_t pb = (snext[w >> 3] >> ((w & 7) * 4)) & 0xFu;  // 4 parents
+            uint32_t mm = 0;
+            mm |= (pb & 1u) ? 0x000000FFu : 0u;
+            mm |= (pb & 2u) ? 0x0000FF00u : 0u;
+            mm |= (pb & 4u) ? 0x00FF0000u : 0u;
+            mm |= (pb & 8u) ? 0xFF000000u : 0u;
+            scur[w] = mm;
+          }
+        }
+        if (tid == 0) { S.scr[SCR_NS] = ns; S.scr[SCR_IP] = ip0; }
       }
+      vcur = vnext;
       uint32_t* tmp = scur;
       scur = snext;
       snext = tmp;
+      A = An;
+    } else if (mode == MODE_COMPACT) {
+      if (staged) out_compact<LT, NT>(olev, N, scur, S.wpre, vcur, S.opv, S.spal, P);
+      else out_compact<LT, NT>(olev, N, scur, S.wpre, vcur, S.opv, pal, P);
+    } else {
+      if (staged) out_global<LT, NT>(olev, N, scur, vcur, S.spal, P);
+      else out_global<LT, NT>(olev, N, scur, vcur, pal, P);
     }
     __syncthreads();
+    if (ended) break;
   }
-  if (status) atomicOr(&S.scr[60], status);
+  if (status) atomicOr(&S.scr[SCR_STATUS], status);
   __syncthreads();
-  if (tid == 0) a.status[r] = S.scr[60];
+  if (tid == 0) a.status[r] = S.scr[SCR_STATUS];
 }
 
-template <typename LT>
-__global__ __launch_bounds__(FCS_OPS_THREADS) void ops_kernel(OpsArgs<LT> a) {
+template <typename LT, class C>
+__global__ __launch_bounds__(C::NT) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
-  OpsShared S;
-  S.bigv = reinterpret_cast<uint16_t*>(smem);
-  S.smv = S.bigv + FCS_LDS_LEVEL_MAX;
-  S.sa = reinterpret_cast<uint32_t*>(S.smv + 4096);
-  S.sb = S.sa + 1024;
-  S.scr = S.sb + 1024;
-  S.act = reinterpret_cast<uint16_t*>(S.scr + 64);
-  __shared__ uint32_t s_req;
+  using Y = Layout<C, LT>;
+  Lds<LT> S;
+  S.sA = reinterpret_cast<uint16_t*>(smem + Y::sA);
+  S.sB = reinterpret_cast<uint16_t*>(smem + Y::sB);
+  S.big = reinterpret_cast<uint16_t*>(smem + Y::big);
+  S.act = reinterpret_cast<uint16_t*>(smem + Y::act);
+  S.wpre = reinterpret_cast<uint16_t*>(smem + Y::wpre);
+  S.opv = reinterpret_cast<uint16_t*>(smem + Y::opv);
+  S.st0 = reinterpret_cast<uint32_t*>(smem + Y::st0);
+  S.st1 = reinterpret_cast<uint32_t*>(smem + Y::st1);
+  S.scr = reinterpret_cast<uint32_t*>(smem + Y::scr);
+  S.spal = reinterpret_cast<LT*>(smem + Y::spal);
   for (;;) {
-    if (threadIdx.x == 0) s_req = atomicAdd(a.counter, 1u);
+    if (threadIdx.x == 0) S.scr[SCR_REQ] = atomicAdd(a.counter, 1u);
     __syncthreads();
-    const uint32_t r = s_req;
+    const uint32_t r = S.scr[SCR_REQ];
     __syncthreads();
     if (r >= a.nreq) break;
-    decode_request<LT>(a, S, r);
+    decode_request<LT, C>(a, S, r);
     __syncthreads();
   }
 }
@@ -398,28 +599,41 @@ __global__ void zero_u32_kernel(uint32_t* p, uint64_t n) {
   if (i < n) p[i] = 0u;
 }
 
+template <typename LT, class C>
+hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out) {
+  constexpr size_t lds = Layout<C, LT>::bytes;
+  static_assert(lds <= 160 * 1024, "op decoder LDS budget");
+  // resident workgroups per CU (queried once per instantiation)
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    const void* fn = reinterpret_cast<const void*>(&ops_kernel<LT, C>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, (int)C::NT, lds);
+    if (e != hipSuccess) return e;
+    per_cu = nb > 0 ? nb : 1;
+  }
+  const uint64_t resident = (uint64_t)per_cu * (num_cu ? num_cu : 1);
+  const uint32_t grid = (uint32_t)(a.nreq < resident ? a.nreq : resident);
+  if (grid_out) *grid_out = grid;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, a.counter, (uint64_t)1);
+  hipLaunchKernelGGL((ops_kernel<LT, C>), dim3(grid), dim3(C::NT), lds, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 size_t fcs_ops_lds_bytes(uint32_t L) {
-  const size_t act_cap = (L >= 6) ? 32768 : 4096;
-  return FCS_LDS_LEVEL_MAX * 2 + 4096 * 2 + 1024 * 4 * 2 + 64 * 4 + act_cap * 2;
+  return L >= 6 ? Layout<CfgLarge, uint64_t>::bytes : Layout<CfgSmall, uint64_t>::bytes;
 }
 
 template <typename LT>
-hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t grid, hipStream_t s) {
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out) {
   if (a.nreq == 0) return hipSuccess;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, a.counter, (uint64_t)1);
-  const size_t lds = fcs_ops_lds_bytes(a.L);
-  static bool attr_set = false;
-  if (!attr_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ops_kernel<LT>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)fcs_ops_lds_bytes(6));
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(ops_kernel<LT>, dim3(grid), dim3(FCS_OPS_THREADS), lds, s, a);
-  return hipGetLastError();
+  if (a.L > 6) return hipErrorInvalidValue;  // 16-bit node indices: bricks up to 64^3
+  if (a.L == 6) return launch_cfg<LT, CfgLarge>(a, num_cu, s, grid_out);
+  return launch_cfg<LT, CfgSmall>(a, num_cu, s, grid_out);
 }
 
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
@@ -428,5 +642,5 @@ hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t);
-template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t);
+template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*);
+template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*);
