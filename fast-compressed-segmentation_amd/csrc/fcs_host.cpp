@@ -64,6 +64,9 @@ struct fcs_volume {
   DecodeReq* d_req = nullptr;
   uint32_t* d_req_status = nullptr;
   size_t req_cap = 0;
+  std::vector<DecodeReq> req_sorted;   // requests in device (cost-binned) order
+  std::vector<uint32_t> perm;            // device position -> caller position
+  std::vector<uint32_t> status_sorted;
   uint32_t* d_meta = nullptr;
   uint64_t* d_offs = nullptr;
   void* d_flabel = nullptr;
@@ -169,7 +172,31 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     return 0;
   }
   if (int rc = ensure_req(v, reqs.size())) return rc;
-  HIP_TRY(c, hipMemcpyAsync(v->d_req, reqs.data(), reqs.size() * sizeof(DecodeReq), hipMemcpyHostToDevice,
+  // Bin the requests by estimated cost, heaviest first (stable within a bin):
+  // the persistent decoder deals the front half round-robin and claims the
+  // rest, so light requests fill the end. Cost ~ nibbles up to the LOD plus
+  // labels written.
+  const size_t n = reqs.size();
+  constexpr int kBins = 48;
+  std::vector<uint32_t> bin(n);
+  std::vector<uint32_t>& perm = v->perm;
+  perm.resize(n);
+  size_t count[kBins + 1] = {0};
+  for (size_t i = 0; i < n; ++i) {
+    const DecodeReq& q = reqs[i];
+    const uint32_t t = q.lod & 0xFF;
+    const uint32_t nsym = q.brick < v->nb ? v->hb[q.brick].nsym : 0;
+    const uint32_t drop = t < v->L ? 3 * (v->L - t) : 0;
+    const uint64_t cost = (uint64_t)(nsym >> (drop < 32 ? drop : 31)) + ((1ull << (3 * (t < 20 ? t : 20))) >> 3) + 1;
+    const int lg = 63 - __builtin_clzll(cost);
+    bin[i] = (uint32_t)(kBins - 1 - (lg < kBins ? lg : kBins - 1));
+    ++count[bin[i] + 1];
+  }
+  for (int b = 0; b < kBins; ++b) count[b + 1] += count[b];
+  for (size_t i = 0; i < n; ++i) perm[count[bin[i]]++] = (uint32_t)i;
+  v->req_sorted.resize(n);
+  for (size_t i = 0; i < n; ++i) v->req_sorted[i] = reqs[perm[i]];
+  HIP_TRY(c, hipMemcpyAsync(v->d_req, v->req_sorted.data(), n * sizeof(DecodeReq), hipMemcpyHostToDevice,
                             c->stream));
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
   if (v->label_bits == 32) {
@@ -182,9 +209,11 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, c->stream));
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-  HIP_TRY(c, hipMemcpyAsync(status.data(), v->d_req_status, reqs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+  v->status_sorted.resize(n);
+  HIP_TRY(c, hipMemcpyAsync(v->status_sorted.data(), v->d_req_status, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; ++i) status[perm[i]] = v->status_sorted[i];
   float ms = 0;
   HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->timing.decode_ms = ms;

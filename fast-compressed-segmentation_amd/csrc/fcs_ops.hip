@@ -31,43 +31,51 @@
 
 namespace {
 
-template <uint32_t NT_, uint32_t VCAP_, uint32_t BIG_, uint32_t ACT_, uint32_t SW_, uint32_t KCAP_, uint32_t PCAP_>
+template <uint32_t NT_, uint32_t VCAP_, uint32_t XCAP_, uint32_t BIG_, uint32_t ACT_, bool PACKED_, uint32_t SW_,
+          uint32_t PBYTES_>
 struct OpsCfg {
-  static constexpr uint32_t NT = NT_;      // threads per workgroup
-  static constexpr uint32_t NW = NT_ / 64;  // waves
-  static constexpr uint32_t VCAP = VCAP_;  // two ping-pong level buffers (u16)
-  static constexpr uint32_t BIG = BIG_;    // level buffer beyond VCAP nodes (u16)
-  static constexpr uint32_t ACT = ACT_;    // active-parent list (u16)
-  static constexpr uint32_t SW = SW_;      // stop-bit words per level
-  static constexpr uint32_t KCAP = KCAP_;  // compact final-level op results (u16)
-  static constexpr uint32_t PCAP = PCAP_;  // palette entries staged in LDS
+  static constexpr uint32_t NT = NT_;        // threads per workgroup
+  static constexpr uint32_t NW = NT_ / 64;    // waves
+  static constexpr uint32_t VCAP = VCAP_;    // two ping-pong level buffers (u16)
+  static constexpr uint32_t XCAP = XCAP_;    // u16 between them: the final level's compact op results use
+                                             // it plus the ping-pong buffer not holding the parents
+  static constexpr uint32_t KCAP = XCAP_ ? VCAP_ + XCAP_ : 0u;
+  static constexpr uint32_t BIG = BIG_;      // level buffer beyond VCAP nodes (u16)
+  static constexpr uint32_t ACT = ACT_;      // active-parent list entries
+  static constexpr bool PACKED = PACKED_;    // active-parent entries carry the parent's value (u32)
+  static constexpr uint32_t SW = SW_;        // stop-bit words per level
+  static constexpr uint32_t PBYTES = PBYTES_;  // palette bytes staged in LDS
 };
-// bricks up to 32^3: four waves, ~46 KB of LDS (three workgroups per CU)
-using CfgSmall = OpsCfg<256, 4096, 0, 4096, 128, 8192, 1024>;
+// bricks up to 32^3: four waves, ~52 KB of LDS (three workgroups per CU for u32 labels)
+using CfgSmall = OpsCfg<256, 4096, 8192, 0, 4096, true, 128, 2048>;
 // 64^3 bricks: sixteen waves, the 32768-node level in LDS (~155 KB)
-using CfgLarge = OpsCfg<1024, 4096, 32768, 32768, 1024, 0, 0>;
+using CfgLarge = OpsCfg<1024, 4096, 0, 32768, 32768, false, 1024, 0>;
 
-constexpr uint32_t SCR_WORDS = 128;
+constexpr uint32_t SCR_WORDS = 256;
 constexpr uint32_t SCR_WAVESUM = 0;   // NW words: active-node counts per wave
 constexpr uint32_t SCR_REC = 16;      // 2 x 48 words: parse records per wave
 constexpr uint32_t SCR_STATUS = 120;
 constexpr uint32_t SCR_NS = 122;      // nibble where the next level starts
 constexpr uint32_t SCR_IP = 123;      // palette cursor at the next level's start
-constexpr uint32_t SCR_REQ = 124;
+constexpr uint32_t SCR_DUM16 = 128;   // 64 u16: write target of lanes without an op
+constexpr uint32_t SCR_DUM32 = 160;   // 64 u32: same, for stop bits
+constexpr uint32_t SCR_IDX = 224;     // 4 words: request index ring
 
 template <class C, typename LT>
 struct Layout {
   static constexpr size_t sA = 0;
-  static constexpr size_t sB = sA + C::VCAP * 2;
+  static constexpr size_t opx = sA + C::VCAP * 2;
+  static constexpr size_t sB = opx + C::XCAP * 2;
   static constexpr size_t big = sB + C::VCAP * 2;
   static constexpr size_t act = big + C::BIG * 2;
-  static constexpr size_t wpre = act + C::ACT * 2;
-  static constexpr size_t opv = (wpre + C::SW * 2 + 15) / 16 * 16;
-  static constexpr size_t st0 = (opv + C::KCAP * 2 + 15) / 16 * 16;
+  static constexpr size_t wpre = act + C::ACT * (C::PACKED ? 4 : 2);
+  static constexpr size_t st0 = (wpre + C::SW * 2 + 15) / 16 * 16;
   static constexpr size_t st1 = st0 + C::SW * 4;
   static constexpr size_t scr = st1 + C::SW * 4;
-  static constexpr size_t spal = (scr + SCR_WORDS * 4 + 15) / 16 * 16;
-  static constexpr size_t bytes = spal + C::PCAP * sizeof(LT);
+  static constexpr size_t ring = scr + SCR_WORDS * 4;  // 4 x (DecodeReq + DevBrick) = 192 bytes
+  static constexpr size_t spal = (ring + 192 + 15) / 16 * 16;
+  static constexpr size_t bytes = spal + C::PBYTES;
+  static constexpr uint32_t PCAP = C::PBYTES / sizeof(LT);
 };
 
 template <typename LT>
@@ -75,12 +83,14 @@ struct Lds {
   uint16_t* sA;
   uint16_t* sB;
   uint16_t* big;
-  uint16_t* act;   // active parents of the level being expanded, Morton order
-  uint16_t* wpre;  // active parents before each stop word (rank lookup)
-  uint16_t* opv;   // compact final-level op results
+  uint32_t* act32;  // active parents of the level being expanded (Morton order): node | value << 16
+  uint16_t* act16;  // same, node only
+  uint16_t* wpre;   // active parents before each stop word (rank lookup)
   uint32_t* st0;
   uint32_t* st1;
   uint32_t* scr;
+  DecodeReq* rq;    // request ring (4)
+  DevBrick* rb;     // brick-record ring (4)
   LT* spal;
 };
 
@@ -88,14 +98,27 @@ enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-template <typename T>
-__device__ __forceinline__ T wave_incl_scan(T v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const T y = __shfl_up(v, d, 64);
-    if (lane >= d) v += y;
-  }
+// Workgroup barrier that orders LDS only: global loads stay in flight across
+// it (__syncthreads() would also wait for every outstanding global access,
+// including prefetches meant for later). Used wherever threads exchange data
+// through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// inclusive wave scan on the VALU data-parallel-primitive lanes: shifts by
+// 1, 2, 4, 8 within each row of 16 lanes, then row 0's / row 1's last lane
+// broadcast into the rows above (row_bcast:15, row_bcast:31). Lanes whose
+// source is outside the row, and rows masked off, add 0.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
   return v;
 }
 
@@ -177,26 +200,27 @@ __device__ __forceinline__ void expand_level(LT* o, uint32_t lt, uint32_t lv, co
   }
 }
 
-// final level from compact op results: 8 labels per parent
+// final level from compact op results: 8 labels per parent. Branch-free so
+// that two parents' LDS reads overlap: stopped parents still read a (valid)
+// op group and then ignore it.
 template <typename LT, uint32_t NT>
-__device__ __forceinline__ void out_compact(LT* olev, uint32_t N, const uint32_t* scur, const uint16_t* wpre,
-                                            const uint16_t* vcur, const uint16_t* opv, const LT* pl, uint32_t P) {
+__device__ __forceinline__ void out_compact(LT* olev, uint32_t N, uint32_t A, const uint32_t* scur,
+                                            const uint16_t* wpre, const uint16_t* vcur, const uint16_t* opv,
+                                            const LT* pl, uint32_t P) {
   const bool al = aligned16(olev);
+#pragma unroll 2
   for (uint32_t m = threadIdx.x; m < N; m += NT) {
+    const uint32_t sw = scur[m >> 5];
+    const bool stp = (sw >> (m & 31)) & 1u;
+    const uint32_t rk = min(wpre[m >> 5] + __popc(~sw & ((1u << (m & 31)) - 1u)), A - 1);
+    const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * rk);
+    const uint32_t pv = vcur[m];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
     LT v8[8];
-    if (is_stopped(scur, m)) {
-      const uint32_t v = vcur[m];
-      const LT u = pl[v < P ? v : P - 1];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) v8[c] = u;
-    } else {
-      const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * rank_of(scur, wpre, m));
-      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t v = (w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
-        v8[c] = pl[v < P ? v : P - 1];
-      }
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t v = stp ? pv : (w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+      v8[c] = pl[v < P ? v : P - 1];
     }
     store8(olev + 8 * m, v8, al);
   }
@@ -225,11 +249,202 @@ __device__ __forceinline__ void out_global(LT* olev, uint32_t N, const uint32_t*
   }
 }
 
-template <typename LT, class C>
-__device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t r) {
+// The brick's nibble stream is parsed in NT-word blocks; each thread keeps
+// its word (and the word after it, for Pd deltas) of the current block and
+// of the next one in registers, so a block's global loads are issued a full
+// parse step before they are used.
+struct Blocks {
+  uint32_t bc;      // block held in w0 / n0
+  uint32_t w0, n0;  // block bc
+  uint32_t w1, n1;  // block bc + 1
+};
+
+template <uint32_t NT>
+__device__ __forceinline__ void load_block(const uint32_t* nibw, uint32_t w_end, uint32_t b, uint32_t& w,
+                                           uint32_t& n) {
+  const uint32_t wi = b * NT + threadIdx.x;
+  w = wi < w_end ? nibw[wi] : 0u;
+  n = wi < w_end ? nibw[wi + 1] : 0u;  // streams carry padding words
+}
+
+// Per-level inputs of the op parser (uniform across the workgroup).
+template <typename LT>
+struct LevelIO {
+  const uint32_t* nibw;
+  uint32_t ns, ne, w_end, K, ip0, P, lmask;
+  const uint16_t* vcur;
+  uint16_t* vnext;   // MODE_LEVEL: next level values
+  uint32_t* snext;   // MODE_LEVEL: next level stop bits
+  uint16_t* opv;     // MODE_COMPACT: per-op results
+  LT* olev;          // MODE_GLOBAL: per-node results in the cache region
+};
+
+// Parse and resolve the ops of one level (P1). Returns the number of ops
+// found (< K means the stream ended early). The lane holding the level's last
+// op publishes where the next level starts (SCR_NS / SCR_IP).
+template <uint32_t MODE, typename LT, class C>
+__device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<LT>& io, Blocks& B, uint32_t& tile,
+                                                uint32_t& status) {
   constexpr uint32_t NT = C::NT, NW = C::NW;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-  const DecodeReq q = a.reqs[r];
+  const uint32_t ns = io.ns, ne = io.ne, K = io.K, P = io.P;
+  uint32_t op_base = 0, pa_base = 0, carry = 0;  // carry: next nibble is a delta
+  uint32_t st = 0;
+  // a level of K ops spans at most 2K nibbles (each op plus a Pd delta):
+  // words past that carry no op of this level and are treated as invalid
+  const uint32_t w_lim = min(io.w_end, ((ns + 2 * K + 7) >> 3) + 1);
+  for (uint32_t b = (ns >> 3) / NT; b * NT < w_lim && op_base < K; ++b, ++tile) {
+    if (b != B.bc) {
+      // levels are contiguous: normally the next block, already in flight
+      if (b == B.bc + 1) { B.w0 = B.w1; B.n0 = B.n1; }
+      else load_block<NT>(io.nibw, io.w_end, b, B.w0, B.n0);
+      B.bc = b;
+      load_block<NT>(io.nibw, io.w_end, b + 1, B.w1, B.n1);
+    }
+    const uint32_t wi = b * NT + tid;
+    const bool inw = wi < w_lim;
+    const uint32_t word = B.w0;
+    const uint32_t wnext = B.n0;
+    const uint32_t g0 = wi * 8;
+    // trailing run of 12/13 nibbles over this lane's valid nibbles, the op /
+    // palette-advance counts for both entering parities, and which nibbles
+    // are ops (bit j) for both parities
+    uint32_t tr = 0, cnt0 = 0, cnt1 = 0, iso0 = 0, iso1 = 0;
+    if (__ballot(inw) != 0) {
+      uint32_t pd0 = 0, pd1 = 1;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t g = g0 + j;
+        const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
+        const bool valid = inw && g >= ns && g < ne;
+        tr = (valid && op == FCS_OP_PD) ? tr + 1 : 0;
+        if (g == ns) { pd0 = 0; pd1 = 0; }
+        const uint32_t inc = 1u | ((op == FCS_OP_PA) ? 0x10000u : 0u);
+        const bool o0 = valid && !pd0, o1 = valid && !pd1;
+        cnt0 += o0 ? inc : 0u;
+        cnt1 += o1 ? inc : 0u;
+        iso0 |= o0 ? (1u << j) : 0u;
+        iso1 |= o1 ? (1u << j) : 0u;
+        pd0 = (o0 && op == FCS_OP_PD) ? 1u : 0u;
+        pd1 = (o1 && op == FCS_OP_PD) ? 1u : 0u;
+      }
+    }
+    const uint64_t Pm = __ballot(tr & 1u);
+    const uint64_t Nm = __ballot(tr != 8);
+    const uint64_t below = Nm & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    const bool fixed = below != 0;  // entering parity set by a lane of this wave
+    const uint32_t pfix = fixed ? (uint32_t)(Pm >> (63 - __clzll(below))) & 1u : 0u;
+    const uint32_t cfix = pfix ? cnt1 : cnt0;
+    const uint32_t v0 = fixed ? cfix : cnt0, v1 = fixed ? cfix : cnt1;
+    const uint32_t i0 = wave_incl_scan(v0), i1 = wave_incl_scan(v1);
+    uint32_t* R = S.scr + SCR_REC + (tile & 1u) * 48u;
+    if (lane == 63) {
+      R[3 * wave] = Nm ? (2u | ((uint32_t)(Pm >> (63 - __clzll(Nm))) & 1u)) : 0u;
+      R[3 * wave + 1] = i0;
+      R[3 * wave + 2] = i1;
+    }
+    lds_barrier();
+    uint32_t X = carry, base = 0, myX = 0, mybase = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+      if (w == wave) { myX = X; mybase = base; }
+      const uint32_t rec = R[3 * w];
+      base += X ? R[3 * w + 2] : R[3 * w + 1];
+      X = (rec & 2u) ? (rec & 1u) : X;
+    }
+    const uint32_t par = fixed ? pfix : myX;
+    const uint32_t ex = mybase + (myX ? i1 - v1 : i0 - v0);
+    // resolve this lane's ops: straight-line code per nibble; lanes without a
+    // live op write to a private dummy slot. Waves without a live op skip.
+    const uint32_t my_ops = (myX ? v1 : v0) & 0xFFFFu;
+    if (__ballot(my_ops != 0 && op_base + (ex & 0xFFFFu) < K) != 0) {
+      uint32_t k = op_base + (ex & 0xFFFFu);
+      uint32_t ipx = io.ip0 + pa_base + (ex >> 16);
+      const uint32_t iso = par ? iso1 : iso0;
+      bool has_end = false;
+      uint32_t end_g = 0, end_ip = 0;
+      uint16_t* dum16 = reinterpret_cast<uint16_t*>(S.scr + SCR_DUM16) + lane;
+      uint32_t* dum32 = S.scr + SCR_DUM32 + lane;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t g = g0 + j;
+        const uint32_t v = (word >> (4 * j)) & 0xFu;
+        const uint32_t op = v >> 1;
+        const bool isop = (iso >> j) & 1u;
+        const bool live = isop && k < K;
+        const uint32_t kk = live ? k : 0u;
+        uint32_t m, pv;
+        if constexpr (C::PACKED) {
+          const uint32_t e = S.act32[kk >> 3];
+          m = e & 0xFFFFu;
+          pv = e >> 16;
+        } else {
+          m = S.act16[kk >> 3];
+          pv = io.vcur[m];
+        }
+        const uint32_t c = kk & 7u, n = (m << 3) | c;
+        // Rx/Ry/Rz: neighbour along the axis by dilated-integer arithmetic on
+        // the Morton code (x-1 for an even coordinate, x+1 for an odd one)
+        const uint32_t ax = (op >= FCS_OP_RX && op <= FCS_OP_RZ) ? op - 1 : 0u;
+        const uint32_t mA = (0x09249249u << ax) & io.lmask;
+        const uint32_t xa = n & mA, rest = n & ~mA;
+        const bool odd = (n >> ax) & 1u;
+        const bool outside = odd ? (xa == mA) : (xa == 0u);
+        const uint32_t nb = (odd ? (((xa | ~mA) + 1u) & mA) : ((xa - 1u) & mA)) | rest;
+        uint32_t nv = io.vcur[nb >> 3];
+        // keep the read unconditional: a branch around it costs more than the read
+        asm volatile("" : "+v"(nv));
+        // outside the brick the reference reads the parent's output slot,
+        // which child 0 has already overwritten when c > 0
+        const uint32_t rval = outside ? ((c == 0) ? pv : (FCS_REF32 | (8 * m)))
+                                      : ((nb < n) ? (FCS_REF32 | nb) : nv);
+        // Pa / P0 / Pd: palette cursor (Pd: delta nibble dn selects ipx - dn - 1)
+        ipx += (live && op == FCS_OP_PA) ? 1u : 0u;
+        const bool isd = (op == FCS_OP_PD);
+        const uint32_t sub = isd ? (((j < 7) ? (word >> (4 * (j + 1))) : wnext) & 0xFu) + 1u : 0u;
+        const uint32_t pidx = ipx - sub;
+        const bool bad_pal = (ipx < sub) || (pidx >= P);
+        const uint32_t pval = bad_pal ? P - 1 : pidx;
+        uint32_t val = (op == FCS_OP_RP) ? pv : (op <= FCS_OP_RZ) ? rval : (op <= FCS_OP_PD) ? pval : 0u;
+        uint32_t sb = (op >= FCS_OP_PA && op <= FCS_OP_PD && bad_pal) ? FCS_ST_PALETTE_RANGE : 0u;
+        sb |= (isd && g + 1 >= ne) ? FCS_ST_STREAM_OVERRUN : 0u;
+        sb |= (op > FCS_OP_PD) ? FCS_ST_BAD_OP : 0u;
+        st |= live ? sb : 0u;
+        const uint32_t v16 = (val & FCS_REF32) ? (FCS_REF16 | (val & 0x7FFFu)) : val;
+        if constexpr (MODE == MODE_LEVEL) {
+          *(live ? &io.vnext[n] : dum16) = (uint16_t)v16;
+          atomicOr(live ? &io.snext[n >> 5] : dum32, (live && (v & 1u)) ? (1u << (n & 31)) : 0u);
+        } else if constexpr (MODE == MODE_COMPACT) {
+          *(live ? &io.opv[k] : dum16) = (uint16_t)v16;
+        } else {
+          if (live) io.olev[n] = (LT)val;
+        }
+        const bool is_end = live && k == K - 1;
+        end_g = is_end ? g + 1 + (isd ? 1u : 0u) : end_g;
+        end_ip = is_end ? ipx : end_ip;
+        has_end = has_end || is_end;
+        k += isop ? 1u : 0u;
+      }
+      if (has_end) {
+        // last op of the level: the next level starts after it (and its delta)
+        S.scr[SCR_NS] = end_g;
+        S.scr[SCR_IP] = end_ip;
+      }
+    }
+    op_base += base & 0xFFFFu;
+    pa_base += base >> 16;
+    carry = X;
+  }
+  status |= st;
+  return op_base;
+}
+
+template <typename LT, class C>
+__device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t r, const DecodeReq& q,
+                               const DevBrick& B) {
+  constexpr uint32_t NT = C::NT, NW = C::NW;
+  constexpr uint32_t PCAP = Layout<C, LT>::PCAP;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const uint32_t b = q.brick, t = q.lod & 0xFFu;
   const bool pyr = (q.lod & FCS_REQ_PYRAMID) != 0;
   const uint32_t L = a.L;
@@ -237,7 +452,6 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     if (tid == 0) a.status[r] = FCS_ST_BAD_REQUEST;
     return;
   }
-  const DevBrick B = a.bricks[b];
   uint32_t status = 0;
   const uint32_t nsym = B.nsym;
   if (nsym == 0) status |= FCS_ST_STREAM_OVERRUN;
@@ -249,6 +463,11 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     return;
   }
   const uint32_t* nibw = a.nib + B.nib_off;
+  const uint32_t w_end = (nsym + 7) >> 3;
+  Blocks blk;
+  blk.bc = 0;
+  load_block<NT>(nibw, w_end, 0, blk.w0, blk.n0);
+  load_block<NT>(nibw, w_end, 1, blk.w1, blk.n1);
   const uint32_t root = nibw[0] & 0xFu;
   const LT p0 = pal[0];
   if ((root & 1u) || t == 0 || (status & FCS_ST_STREAM_OVERRUN)) {
@@ -257,7 +476,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     if (tid == 0) a.status[r] = status;
     return;
   }
-  const bool staged = C::PCAP != 0 && P <= C::PCAP;
+  const bool staged = PCAP != 0 && P <= PCAP;
   if (staged)
     for (uint32_t i = tid; i < P; i += NT) S.spal[i] = pal[i];
   const LT* lp = staged ? static_cast<const LT*>(S.spal) : pal;
@@ -265,7 +484,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   if (tid == 0) {
     // level 0: the root (palette index 0, active); level 1 not stopped yet
     S.sA[0] = 0;
-    S.act[0] = 0;
+    if constexpr (C::PACKED) S.act32[0] = 0; else S.act16[0] = 0;
     S.wpre[0] = 0;
     S.st0[0] = 0;
     S.st1[0] = 0;
@@ -273,7 +492,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     S.scr[SCR_NS] = 1;
     S.scr[SCR_IP] = 0;
   }
-  __syncthreads();
+  lds_barrier();
 
   uint16_t* vcur = S.sA;
   uint32_t* scur = S.st0;
@@ -281,8 +500,6 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   // level l's ops start at nibble ns with palette cursor ip0 (nibble 0 is the
   // root op); each level's end is found while parsing it
   uint32_t A = 1, ns = 1, ip0 = 0, tile = 0;
-  const uint32_t w_end = (nsym + 7) >> 3;
-  const uint32_t ne = nsym;  // parse bound; a level ends after its K-th op
 
   for (uint32_t l = 0; l < t; ++l) {
     const uint32_t N = 1u << (3 * l), Cn = N << 3, K = 8 * A;
@@ -290,6 +507,8 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     const uint32_t mode = !last ? MODE_LEVEL : ((C::KCAP != 0 && K <= C::KCAP) ? MODE_COMPACT : MODE_GLOBAL);
     LT* olev = o + (pyr ? fcs_pyramid_level_offset(l + 1) : 0ull);
     uint16_t* vnext = (Cn <= C::VCAP) ? (vcur == S.sA ? S.sB : S.sA) : S.big;
+    // compact op results: the free ping-pong buffer and the area next to it
+    uint16_t* opv = (vcur == S.sA) ? S.sA + C::VCAP : S.sA;
 
     // ---- P1: stopped children; parse and resolve the level's ops
     if (mode == MODE_LEVEL) {
@@ -299,135 +518,13 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
           *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(v, v, v, v);
         }
     }
-    uint32_t op_base = 0, pa_base = 0, carry = 0;  // carry: next nibble is a delta
-    const uint32_t side = 1u << (l + 1);
-    for (uint32_t wt = ns >> 3; wt < w_end && op_base < K; wt += NT, ++tile) {
-      const uint32_t wi = wt + tid;
-      const bool inw = wi < w_end;
-      const uint32_t word = inw ? nibw[wi] : 0u;
-      const uint32_t g0 = wi * 8;
-      // trailing run of 12/13 nibbles over this lane's valid nibbles, and the
-      // op / palette-advance counts for both entering parities
-      uint32_t tr = 0, cnt0 = 0, cnt1 = 0;
-      {
-        uint32_t pd0 = 0, pd1 = 1;
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) {
-          const uint32_t g = g0 + j;
-          const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
-          const bool valid = inw && g >= ns && g < ne;
-          tr = (valid && op == FCS_OP_PD) ? tr + 1 : 0;
-          if (g == ns) { pd0 = 0; pd1 = 0; }
-          const uint32_t inc = 1u | ((op == FCS_OP_PA) ? 0x10000u : 0u);
-          if (valid) {
-            if (pd0) pd0 = 0; else { cnt0 += inc; pd0 = (op == FCS_OP_PD); }
-            if (pd1) pd1 = 0; else { cnt1 += inc; pd1 = (op == FCS_OP_PD); }
-          }
-        }
-      }
-      const uint64_t Pm = __ballot(tr & 1u);
-      const uint64_t Nm = __ballot(tr != 8);
-      const uint64_t below = Nm & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-      const bool fixed = below != 0;  // entering parity set by a lane of this wave
-      const uint32_t pfix = fixed ? (uint32_t)(Pm >> (63 - __clzll(below))) & 1u : 0u;
-      const uint32_t cfix = pfix ? cnt1 : cnt0;
-      const uint32_t v0 = fixed ? cfix : cnt0, v1 = fixed ? cfix : cnt1;
-      const uint32_t i0 = wave_incl_scan(v0), i1 = wave_incl_scan(v1);
-      uint32_t* R = S.scr + SCR_REC + (tile & 1u) * 48u;
-      if (lane == 63) {
-        R[3 * wave] = Nm ? (2u | ((uint32_t)(Pm >> (63 - __clzll(Nm))) & 1u)) : 0u;
-        R[3 * wave + 1] = i0;
-        R[3 * wave + 2] = i1;
-      }
-      __syncthreads();
-      uint32_t X = carry, base = 0, myX = 0, mybase = 0;
-#pragma unroll
-      for (uint32_t w = 0; w < NW; ++w) {
-        if (w == wave) { myX = X; mybase = base; }
-        const uint32_t rec = R[3 * w];
-        base += X ? R[3 * w + 2] : R[3 * w + 1];
-        X = (rec & 2u) ? (rec & 1u) : X;
-      }
-      const uint32_t par = fixed ? pfix : myX;
-      const uint32_t ex = mybase + (myX ? i1 - v1 : i0 - v0);
-      // resolve this lane's ops
-      {
-        uint32_t k = op_base + (ex & 0xFFFFu);
-        uint32_t ipx = ip0 + pa_base + (ex >> 16);
-        uint32_t pd = par;
-        for (uint32_t j = 0; j < 8; ++j) {
-          const uint32_t g = g0 + j;
-          if (!inw || g >= ne) break;
-          if (g < ns) continue;
-          if (g == ns) pd = 0;
-          const uint32_t v = (word >> (4 * j)) & 0xFu;
-          if (pd) { pd = 0; continue; }
-          const uint32_t op = v >> 1, stop = v & 1u;
-          pd = (op == FCS_OP_PD);
-          if (k >= K) break;  // the rest belongs to the next level
-          const uint32_t m = S.act[k >> 3], c = k & 7u, n = 8 * m + c;
-          uint32_t val;  // palette index, or FCS_REF32 | node
-          switch (op) {
-            case FCS_OP_RP: val = vcur[m]; break;
-            case FCS_OP_RX: case FCS_OP_RY: case FCS_OP_RZ: {
-              uint32_t x = fcs_unmorton_x(n), y = fcs_unmorton_y(n), z = fcs_unmorton_z(n);
-              uint32_t& qa = (op == FCS_OP_RX) ? x : ((op == FCS_OP_RY) ? y : z);
-              const bool odd = qa & 1u;
-              if ((!odd && qa == 0) || (odd && qa + 1 >= side)) {
-                // outside the brick: the reference reads the parent's output
-                // slot, which child 0 has already overwritten when c > 0
-                val = (c == 0) ? (uint32_t)vcur[m] : (FCS_REF32 | (8 * m));
-              } else {
-                qa = odd ? qa + 1 : qa - 1;
-                const uint32_t nb = fcs_morton3(x, y, z);
-                val = (nb < n) ? (FCS_REF32 | nb) : (uint32_t)vcur[nb >> 3];
-              }
-              break;
-            }
-            case FCS_OP_PA:
-              ++ipx;
-              val = ipx;
-              if (ipx >= P) { status |= FCS_ST_PALETTE_RANGE; val = P - 1; }
-              break;
-            case FCS_OP_P0:
-              val = ipx;
-              if (ipx >= P) { status |= FCS_ST_PALETTE_RANGE; val = P - 1; }
-              break;
-            case FCS_OP_PD: {
-              if (g + 1 >= nsym) status |= FCS_ST_STREAM_OVERRUN;
-              const uint32_t dn = (j < 7) ? ((word >> (4 * (j + 1))) & 0xFu) : (nibw[wi + 1] & 0xFu);
-              const int64_t idx = (int64_t)ipx - (int64_t)dn - 1;
-              if (idx < 0 || idx >= (int64_t)P) { status |= FCS_ST_PALETTE_RANGE; val = P - 1; }
-              else val = (uint32_t)idx;
-              break;
-            }
-            default:
-              status |= FCS_ST_BAD_OP;
-              val = 0;
-          }
-          const uint32_t v16 = (val & FCS_REF32) ? (FCS_REF16 | (val & 0x7FFFu)) : val;
-          if (mode == MODE_LEVEL) {
-            vnext[n] = (uint16_t)v16;
-            if (stop) atomicOr(&snext[n >> 5], 1u << (n & 31));
-          } else if (mode == MODE_COMPACT) {
-            S.opv[k] = (uint16_t)v16;
-          } else {
-            olev[n] = (LT)val;
-          }
-          if (k == K - 1) {
-            // last op of the level: the next level starts after it (and its delta)
-            S.scr[SCR_NS] = g + 1 + (op == FCS_OP_PD ? 1u : 0u);
-            S.scr[SCR_IP] = ipx;
-          }
-          ++k;
-        }
-      }
-      op_base += base & 0xFFFFu;
-      pa_base += base >> 16;
-      carry = X;
-    }
-    if (op_base < K) status |= FCS_ST_STREAM_OVERRUN | FCS_ST_NSYM_MISMATCH;
-    __syncthreads();
+    LevelIO<LT> io{nibw, ns, nsym, w_end, K, ip0, P, (1u << (3 * (l + 1))) - 1u, vcur, vnext, snext, opv, olev};
+    uint32_t found;
+    if (mode == MODE_LEVEL) found = parse_level<MODE_LEVEL, LT, C>(S, io, blk, tile, status);
+    else if (mode == MODE_COMPACT) found = parse_level<MODE_COMPACT, LT, C>(S, io, blk, tile, status);
+    else found = parse_level<MODE_GLOBAL, LT, C>(S, io, blk, tile, status);
+    if (found < K) status |= FCS_ST_STREAM_OVERRUN | FCS_ST_NSYM_MISMATCH;
+    if (mode == MODE_GLOBAL) __syncthreads(); else lds_barrier();  // per-node results in the cache region
     ns = S.scr[SCR_NS];
     ip0 = S.scr[SCR_IP];
     if (l + 1 == L && ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
@@ -435,21 +532,30 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     // ---- P2: chase references. A valid reference points to a smaller node,
     // so a chain strictly decreases; anything else is corrupt input. Only op
     // children can hold references.
+    auto act_node = [&](uint32_t k) -> uint32_t {
+      if constexpr (C::PACKED) return S.act32[k >> 3] & 0xFFFFu; else return S.act16[k >> 3];
+    };
     uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
     const uint32_t nw = (Cn + 31) >> 5;
     if (mode == MODE_LEVEL) {
-      for (uint32_t k = tid; k < K; k += NT) {
-        const uint32_t n = 8 * S.act[k >> 3] + (k & 7u);
-        uint32_t v = vnext[n];
-        if (v & FCS_REF16) {
-          uint32_t cur = n;
-          do {
-            const uint32_t nx = v & 0x7FFFu;
-            if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-            cur = nx;
-            v = vnext[nx];
-          } while (v & FCS_REF16);
-          vnext[n] = (uint16_t)v;
+      // one active parent's 8 children per step (one 16-byte LDS read)
+      for (uint32_t gp = tid; gp < A; gp += NT) {
+        const uint32_t n0 = 8 * act_node(8 * gp);
+        const uint4 q = *reinterpret_cast<const uint4*>(vnext + n0);
+        if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
+          for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t n = n0 + c;
+            uint32_t v = vnext[n];
+            if (!(v & FCS_REF16)) continue;
+            uint32_t cur = n;
+            do {
+              const uint32_t nx = v & 0x7FFFu;
+              if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
+              cur = nx;
+              v = vnext[nx];
+            } while (v & FCS_REF16);
+            vnext[n] = (uint16_t)v;
+          }
         }
       }
       // active nodes of level l+1, over consecutive stop words per thread
@@ -461,25 +567,32 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
       inc = wave_incl_scan(cnt);
       if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
     } else if (mode == MODE_COMPACT) {
-      for (uint32_t k = tid; k < K; k += NT) {
-        uint32_t v = S.opv[k];
-        if (v & FCS_REF16) {
-          uint32_t cur = 8 * S.act[k >> 3] + (k & 7u);
-          for (;;) {
-            const uint32_t nb = v & 0x7FFFu;
-            if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-            const uint32_t p = nb >> 3;
-            if (is_stopped(scur, p)) { v = vcur[p]; break; }
-            v = S.opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
-            if (!(v & FCS_REF16)) break;
-            cur = nb;
+      // one active parent's 8 op results per step (one 16-byte LDS read)
+      for (uint32_t gp = tid; gp < A; gp += NT) {
+        const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
+        if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
+          const uint32_t n0 = 8 * act_node(8 * gp);
+          for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t k = 8 * gp + c;
+            uint32_t v = opv[k];
+            if (!(v & FCS_REF16)) continue;
+            uint32_t cur = n0 + c;
+            for (;;) {
+              const uint32_t nb = v & 0x7FFFu;
+              if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
+              const uint32_t p = nb >> 3;
+              if (is_stopped(scur, p)) { v = vcur[p]; break; }
+              v = opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
+              if (!(v & FCS_REF16)) break;
+              cur = nb;
+            }
+            opv[k] = (uint16_t)v;
           }
-          S.opv[k] = (uint16_t)v;
         }
       }
     } else {
       for (uint32_t k = tid; k < K; k += NT) {
-        const uint32_t n = 8 * S.act[k >> 3] + (k & 7u);
+        const uint32_t n = 8 * act_node(k) + (k & 7u);
         uint32_t v = (uint32_t)olev[n];
         if (v & FCS_REF32) {
           uint32_t cur = n;
@@ -496,7 +609,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
         }
       }
     }
-    __syncthreads();
+    if (mode == MODE_GLOBAL) __syncthreads(); else lds_barrier();
 
     // ---- P3: next level's active list; labels out
     bool ended = false;
@@ -531,7 +644,9 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
           while (bits) {
             const uint32_t j = __ffs(bits) - 1;
             bits &= bits - 1;
-            S.act[k0++] = (uint16_t)(w * 32 + j);
+            const uint32_t m = w * 32 + j;
+            if constexpr (C::PACKED) S.act32[k0++] = m | ((uint32_t)vnext[m] << 16);
+            else S.act16[k0++] = (uint16_t)m;
           }
         }
         if (l + 2 < t) {
@@ -554,20 +669,30 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
       snext = tmp;
       A = An;
     } else if (mode == MODE_COMPACT) {
-      if (staged) out_compact<LT, NT>(olev, N, scur, S.wpre, vcur, S.opv, S.spal, P);
-      else out_compact<LT, NT>(olev, N, scur, S.wpre, vcur, S.opv, pal, P);
+      if (staged) out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, S.spal, P);
+      else out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, pal, P);
     } else {
       if (staged) out_global<LT, NT>(olev, N, scur, vcur, S.spal, P);
       else out_global<LT, NT>(olev, N, scur, vcur, pal, P);
     }
-    __syncthreads();
+    lds_barrier();
     if (ended) break;
   }
   if (status) atomicOr(&S.scr[SCR_STATUS], status);
-  __syncthreads();
+  lds_barrier();
   if (tid == 0) a.status[r] = S.scr[SCR_STATUS];
 }
 
+// Persistent workgroups drain the request list, which the host orders by
+// estimated cost, heaviest first. The first half is dealt round-robin (equal
+// shares, since neighbouring requests cost about the same); the rest is
+// claimed through one atomic counter to even out the finishing times. (One
+// counter shared by every workgroup serialises: claiming every request
+// through it caps a launch of cheap requests at the counter's atomic rate.) Request
+// metadata is fetched ahead through 4-entry rings in LDS so that its latency
+// overlaps the previous request's decode: while request i decodes, thread 0
+// takes the index of request i+3, loads the DecodeReq of request i+2 and the
+// brick record of request i+1; they are stored to LDS at the end of request i.
 template <typename LT, class C>
 __global__ __launch_bounds__(C::NT) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -576,21 +701,64 @@ __global__ __launch_bounds__(C::NT) void ops_kernel(OpsArgs<LT> a) {
   S.sA = reinterpret_cast<uint16_t*>(smem + Y::sA);
   S.sB = reinterpret_cast<uint16_t*>(smem + Y::sB);
   S.big = reinterpret_cast<uint16_t*>(smem + Y::big);
-  S.act = reinterpret_cast<uint16_t*>(smem + Y::act);
+  S.act32 = reinterpret_cast<uint32_t*>(smem + Y::act);
+  S.act16 = reinterpret_cast<uint16_t*>(smem + Y::act);
   S.wpre = reinterpret_cast<uint16_t*>(smem + Y::wpre);
-  S.opv = reinterpret_cast<uint16_t*>(smem + Y::opv);
   S.st0 = reinterpret_cast<uint32_t*>(smem + Y::st0);
   S.st1 = reinterpret_cast<uint32_t*>(smem + Y::st1);
   S.scr = reinterpret_cast<uint32_t*>(smem + Y::scr);
+  S.rq = reinterpret_cast<DecodeReq*>(smem + Y::ring);
+  S.rb = reinterpret_cast<DevBrick*>(smem + Y::ring + 64);
   S.spal = reinterpret_cast<LT*>(smem + Y::spal);
-  for (;;) {
-    if (threadIdx.x == 0) S.scr[SCR_REQ] = atomicAdd(a.counter, 1u);
-    __syncthreads();
-    const uint32_t r = S.scr[SCR_REQ];
-    __syncthreads();
-    if (r >= a.nreq) break;
-    decode_request<LT, C>(a, S, r);
-    __syncthreads();
+  const uint32_t nreq = a.nreq;
+  const DecodeReq none{0xFFFFFFFFu, 0u, 0ull};
+  const DevBrick nobrick{0ull, 0ull, 0u, 0u, 0u, 0u};
+  const uint32_t G = gridDim.x;
+  const uint32_t n_static = (nreq / 2) / G * G;
+  uint32_t sj = 0;  // thread 0: round-robin requests taken so far
+  auto next_index = [&]() -> uint32_t {
+    const uint32_t rr = blockIdx.x + sj * G;
+    if (rr < n_static) { ++sj; return rr; }
+    return n_static + atomicAdd(a.counter, 1u);
+  };
+  auto fetch_req = [&](uint32_t i) -> DecodeReq { return i < nreq ? a.reqs[i] : none; };
+  auto fetch_brick = [&](const DecodeReq& q) -> DevBrick { return q.brick < a.nbricks ? a.bricks[q.brick] : nobrick; };
+  if (threadIdx.x == 0) {
+    const uint32_t r0 = next_index();
+    const uint32_t r1 = next_index();
+    const uint32_t r2 = next_index();
+    const DecodeReq q0 = fetch_req(r0), q1 = fetch_req(r1);
+    const DevBrick b0 = fetch_brick(q0);
+    S.scr[SCR_IDX + 0] = r0;
+    S.scr[SCR_IDX + 1] = r1;
+    S.scr[SCR_IDX + 2] = r2;
+    S.rq[0] = q0;
+    S.rq[1] = q1;
+    S.rb[0] = b0;
+  }
+  lds_barrier();
+  for (uint32_t i = 0;; ++i) {
+    const uint32_t s0 = i & 3u;
+    const uint32_t r = S.scr[SCR_IDX + s0];
+    if (r >= nreq) break;
+    const DecodeReq q = S.rq[s0];
+    const DevBrick B = S.rb[s0];
+    // prefetch (thread 0): queue index for i+3, request i+2, brick record i+1
+    uint32_t r3 = 0;
+    DecodeReq q2 = none;
+    DevBrick b1 = nobrick;
+    if (threadIdx.x == 0) {
+      r3 = next_index();
+      q2 = fetch_req(S.scr[SCR_IDX + ((i + 2) & 3u)]);
+      b1 = fetch_brick(S.rq[(i + 1) & 3u]);
+    }
+    decode_request<LT, C>(a, S, r, q, B);
+    if (threadIdx.x == 0) {
+      S.scr[SCR_IDX + ((i + 3) & 3u)] = r3;
+      S.rq[(i + 2) & 3u] = q2;
+      S.rb[(i + 1) & 3u] = b1;
+    }
+    lds_barrier();
   }
 }
 
