@@ -11,24 +11,31 @@
 #include <vector>
 
 #include "fcs/fcs.h"
+#include "fcs/fcs_mlp.h"
 #include "fcs_common.h"
 #include "fcs_device.h"
+#include "fcs_internal.h"
 #include "fcs_kernels.h"
 
-struct fcs_ctx {
-  int device = 0;
-  int num_cu = 256;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  std::string err;
-  fcs_timing timing{};
-  int rans_mapping = FCS_RANS_MAPPING_AUTO;
-};
 
 namespace {
 
 struct Alloc {
   uint64_t off = 0, size = 0;
+};
+
+// Device request / status buffers of one stream's decode launches, and the
+// host copies of its last launch (requests in cost-binned order, the
+// permutation back to the caller's order, statuses in device order).
+struct DecodeLane {
+  DecodeReq* d_req = nullptr;
+  uint32_t* d_status = nullptr;
+  uint32_t* d_counter = nullptr;
+  size_t cap = 0;
+  std::vector<DecodeReq> sorted;
+  std::vector<uint32_t> perm;
+  std::vector<uint32_t> status_sorted;
+  std::vector<DecodeReq> reqs;  // caller order
 };
 
 }  // namespace
@@ -60,13 +67,13 @@ struct fcs_volume {
   std::vector<int16_t> res_lod;   // -1 = not resident
   std::vector<uint8_t> res_pyr;   // 1 = pyramid layout (levels 0..res_lod)
   std::vector<Alloc> res_alloc;
-  // staging
-  DecodeReq* d_req = nullptr;
-  uint32_t* d_req_status = nullptr;
-  size_t req_cap = 0;
-  std::vector<DecodeReq> req_sorted;   // requests in device (cost-binned) order
-  std::vector<uint32_t> perm;            // device position -> caller position
-  std::vector<uint32_t> status_sorted;
+  // direct-mapped cache: brick i owns voxels [i * stride, (i + 1) * stride)
+  bool direct = false;
+  uint64_t stride = 0;
+  // decode launches: `lane_main` on the context stream (synchronous API
+  // calls), `lane_aux` on the aux stream (prefetches, finished later)
+  DecodeLane lane_main, lane_aux;
+  std::vector<uint8_t> aux_lod;  // LOD of each pending prefetch request
   uint32_t* d_meta = nullptr;
   uint64_t* d_offs = nullptr;
   void* d_flabel = nullptr;
@@ -74,24 +81,18 @@ struct fcs_volume {
   float* d_ft = nullptr;
   uint32_t* d_visited = nullptr;
   uint32_t* d_misses = nullptr;
+  unsigned long long* d_csum = nullptr;
   size_t frame_cap = 0;
+  size_t frame_px = 0;  // pixels of the last rendered frame
 
   size_t lbytes() const { return label_bits / 8; }
 };
 
 namespace {
 
-int fail(fcs_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
+int fail(fcs_ctx* c, int code, const std::string& msg) { return fcs_fail(c, code, msg); }
 
-#define HIP_TRY(ctx, expr)                                                              \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail((ctx), FCS_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));   \
-  } while (0)
+#define HIP_TRY(ctx, expr) FCS_HIP_TRY(ctx, expr)
 
 uint32_t rd32(const uint8_t* p) {
   uint32_t v;
@@ -108,7 +109,12 @@ int size_class(uint64_t voxels) {
   return -1;
 }
 
-bool cache_alloc(fcs_volume* v, uint64_t voxels, uint64_t* off) {
+bool cache_alloc(fcs_volume* v, uint32_t brick, uint64_t voxels, uint64_t* off) {
+  if (v->direct) {
+    if (voxels > v->stride) return false;
+    *off = v->stride * brick;
+    return true;
+  }
   const int cls = size_class(voxels);
   if (cls >= 0 && (size_t)cls < v->free_lists.size() && !v->free_lists[cls].empty()) {
     *off = v->free_lists[cls].back();
@@ -123,12 +129,21 @@ bool cache_alloc(fcs_volume* v, uint64_t voxels, uint64_t* off) {
   return true;
 }
 
+bool resident_at(const fcs_volume* v, uint32_t i, uint32_t t) {
+  const int r = v->res_lod[i];
+  return r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= t) || (!v->res_pyr[i] && (uint32_t)r == t));
+}
+
 // Releases are deferred to the end of a batch: a region dropped while the
 // batch is being built may still be the target of an earlier request of the
 // same launch, so it must not be handed to another request of that launch.
 void cache_release(fcs_volume* v, uint32_t brick, std::vector<Alloc>* deferred) {
   if (v->res_lod[brick] < 0) return;
   const Alloc a = v->res_alloc[brick];
+  if (v->direct) {
+    v->res_lod[brick] = -1;
+    return;
+  }
   if (deferred) {
     deferred->push_back(a);
   } else {
@@ -145,16 +160,25 @@ void cache_free_deferred(fcs_volume* v, const std::vector<Alloc>& deferred) {
   }
 }
 
-int ensure_req(fcs_volume* v, size_t n) {
-  if (n <= v->req_cap) return 0;
+int ensure_req(fcs_volume* v, DecodeLane& ln, size_t n) {
   fcs_ctx* c = v->ctx;
-  if (v->d_req) { (void)hipFree(v->d_req); v->d_req = nullptr; }
-  if (v->d_req_status) { (void)hipFree(v->d_req_status); v->d_req_status = nullptr; }
+  if (!ln.d_counter) HIP_TRY(c, hipMalloc(&ln.d_counter, 64));
+  if (n <= ln.cap) return 0;
+  if (ln.d_req) { (void)hipFree(ln.d_req); ln.d_req = nullptr; }
+  if (ln.d_status) { (void)hipFree(ln.d_status); ln.d_status = nullptr; }
+  ln.cap = 0;
   size_t cap = std::max<size_t>(n, 1024);
-  HIP_TRY(c, hipMalloc(&v->d_req, cap * sizeof(DecodeReq)));
-  HIP_TRY(c, hipMalloc(&v->d_req_status, cap * sizeof(uint32_t)));
-  v->req_cap = cap;
+  HIP_TRY(c, hipMalloc(&ln.d_req, cap * sizeof(DecodeReq)));
+  HIP_TRY(c, hipMalloc(&ln.d_status, cap * sizeof(uint32_t)));
+  ln.cap = cap;
   return 0;
+}
+
+void free_lane(DecodeLane& ln) {
+  if (ln.d_req) (void)hipFree(ln.d_req);
+  if (ln.d_status) (void)hipFree(ln.d_status);
+  if (ln.d_counter) (void)hipFree(ln.d_counter);
+  ln = DecodeLane{};
 }
 
 int ensure_cache(fcs_volume* v) {
@@ -163,24 +187,24 @@ int ensure_cache(fcs_volume* v) {
 }
 
 // Launch the op decoder over `reqs`; statuses copied back.
-int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
-               uint64_t* voxels_out) {
+// Launch the op decoder over `reqs` on `stream` through lane `ln`; the
+// statuses are copied back asynchronously (read them after the stream is
+// synchronised, with finish_decode). With `timed`, the launch is bracketed by
+// the context's timing events.
+int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, hipStream_t stream,
+                  bool timed) {
   fcs_ctx* c = v->ctx;
-  status.assign(reqs.size(), 0);
-  if (reqs.empty()) {
-    c->timing.decode_ms = 0.0f;
-    return 0;
-  }
-  if (int rc = ensure_req(v, reqs.size())) return rc;
+  ln.reqs = reqs;
+  const size_t n = reqs.size();
+  if (n == 0) return 0;
+  if (int rc = ensure_req(v, ln, n)) return rc;
   // Bin the requests by estimated cost, heaviest first (stable within a bin):
   // the persistent decoder deals the front half round-robin and claims the
   // rest, so light requests fill the end. Cost ~ nibbles up to the LOD plus
   // labels written.
-  const size_t n = reqs.size();
   constexpr int kBins = 48;
   std::vector<uint32_t> bin(n);
-  std::vector<uint32_t>& perm = v->perm;
-  perm.resize(n);
+  ln.perm.resize(n);
   size_t count[kBins + 1] = {0};
   for (size_t i = 0; i < n; ++i) {
     const DecodeReq& q = reqs[i];
@@ -193,27 +217,47 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     ++count[bin[i] + 1];
   }
   for (int b = 0; b < kBins; ++b) count[b + 1] += count[b];
-  for (size_t i = 0; i < n; ++i) perm[count[bin[i]]++] = (uint32_t)i;
-  v->req_sorted.resize(n);
-  for (size_t i = 0; i < n; ++i) v->req_sorted[i] = reqs[perm[i]];
-  HIP_TRY(c, hipMemcpyAsync(v->d_req, v->req_sorted.data(), n * sizeof(DecodeReq), hipMemcpyHostToDevice,
-                            c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  for (size_t i = 0; i < n; ++i) ln.perm[count[bin[i]]++] = (uint32_t)i;
+  ln.sorted.resize(n);
+  for (size_t i = 0; i < n; ++i) ln.sorted[i] = reqs[ln.perm[i]];
+  HIP_TRY(c, hipMemcpyAsync(ln.d_req, ln.sorted.data(), n * sizeof(DecodeReq), hipMemcpyHostToDevice, stream));
+  if (timed) HIP_TRY(c, hipEventRecord(c->ev[0], stream));
   if (v->label_bits == 32) {
-    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, v->d_req,
-                        (uint32_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, c->stream));
+    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, ln.d_req,
+                        (uint32_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, stream));
   } else {
-    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, v->d_req,
-                        (uint64_t*)v->d_cache, v->d_req_status, v->d_counter, (uint32_t)reqs.size(), v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, c->stream));
+    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, ln.d_req,
+                        (uint64_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, stream));
   }
-  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-  v->status_sorted.resize(n);
-  HIP_TRY(c, hipMemcpyAsync(v->status_sorted.data(), v->d_req_status, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            c->stream));
+  if (timed) HIP_TRY(c, hipEventRecord(c->ev[1], stream));
+  ln.status_sorted.resize(n);
+  HIP_TRY(c, hipMemcpyAsync(ln.status_sorted.data(), ln.d_status, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream));
+  return 0;
+}
+
+// Statuses of lane `ln`'s last launch in the caller's order (the launch's
+// stream must have been synchronised).
+void finish_decode(DecodeLane& ln, std::vector<uint32_t>& status) {
+  const size_t n = ln.reqs.size();
+  status.assign(n, 0);
+  for (size_t i = 0; i < n; ++i) status[ln.perm[i]] = ln.status_sorted[i];
+}
+
+// Decode `reqs` on the context stream and wait; statuses in caller order.
+int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
+               uint64_t* voxels_out) {
+  fcs_ctx* c = v->ctx;
+  status.assign(reqs.size(), 0);
+  if (reqs.empty()) {
+    c->timing.decode_ms = 0.0f;
+    return 0;
+  }
+  if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true)) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < n; ++i) status[perm[i]] = v->status_sorted[i];
+  finish_decode(v->lane_main, status);
   float ms = 0;
   HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->timing.decode_ms = ms;
@@ -231,6 +275,63 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
   c->timing.decode_bytes = bytes;
   if (voxels_out) *voxels_out = vox;
   return 0;
+}
+
+// Wait for the prefetch launched on the aux stream (if any) and settle its
+// statuses: a failed prefetch leaves its brick non-resident, so the next
+// on-demand pass decodes it again and reports the failure.
+int settle_prefetch(fcs_volume* v) {
+  fcs_ctx* c = v->ctx;
+  if (!c->aux_pending) return 0;
+  HIP_TRY(c, hipEventSynchronize(c->aux_done));
+  c->aux_pending = false;
+  std::vector<uint32_t> st;
+  finish_decode(v->lane_aux, st);
+  for (size_t i = 0; i < st.size(); ++i)
+    if (st[i]) cache_release(v, v->lane_aux.reqs[i].brick, nullptr);
+  v->lane_aux.reqs.clear();
+  return 0;
+}
+
+// Bricks a ray of the frame can enter. Perspective: the brick box (grown by
+// one voxel against rounding) is tested against the four side planes through
+// the camera position and the plane behind it; a box wholly outside one plane
+// is culled. Orthographic frames keep every brick.
+std::vector<uint8_t> frustum_mask(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint32_t h) {
+  std::vector<uint8_t> vis(v->nb, 1);
+  if (cam->ortho_half_height > 0.0f || !(cam->tan_half_fov > 0.0f)) return vis;
+  const double a = (double)w / (double)h, T = cam->tan_half_fov;
+  double p[3], f[3], r[3], u[3];
+  for (int k = 0; k < 3; ++k) {
+    p[k] = cam->position[k];
+    f[k] = cam->forward[k];
+    r[k] = cam->right[k];
+    u[k] = cam->up[k];
+  }
+  // inside: n . (x - p) <= 0 for every plane normal n
+  double n[5][3];
+  for (int k = 0; k < 3; ++k) {
+    n[0][k] = r[k] - a * T * f[k];
+    n[1][k] = -r[k] - a * T * f[k];
+    n[2][k] = u[k] - T * f[k];
+    n[3][k] = -u[k] - T * f[k];
+    n[4][k] = -f[k];
+  }
+  const double b = v->b;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    const uint32_t bx = fcs_unmorton_x(i), by = fcs_unmorton_y(i), bz = fcs_unmorton_z(i);
+    const double lo[3] = {bx * b - 1.0, by * b - 1.0, bz * b - 1.0};
+    const double hi[3] = {(bx + 1) * b + 1.0, (by + 1) * b + 1.0, (bz + 1) * b + 1.0};
+    for (int q = 0; q < 5; ++q) {
+      double d = 0;
+      for (int k = 0; k < 3; ++k) d += n[q][k] * ((n[q][k] >= 0 ? lo[k] : hi[k]) - p[k]);
+      if (d > 0) {
+        vis[i] = 0;
+        break;
+      }
+    }
+  }
+  return vis;
 }
 
 template <typename T>
@@ -258,7 +359,12 @@ int fcs_create(int hip_device, fcs_ctx** out) {
   if (hip_device < 0 || hip_device >= n) return FCS_E_INVALID;
   fcs_ctx* c = new fcs_ctx();
   c->device = hip_device;
-  if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->aux_done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->main_mark, hipEventDisableTiming) != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     delete c;
     return FCS_E_HIP;
   }
@@ -272,8 +378,13 @@ int fcs_create(int hip_device, fcs_ctx** out) {
 void fcs_destroy(fcs_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  while (!c->volumes.empty()) fcs_volume_release_device(c->volumes.back());
+  while (!c->mlps.empty()) fcs_mlp_release_device(c->mlps.back());
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->aux_done) (void)hipEventDestroy(c->aux_done);
+  if (c->main_mark) (void)hipEventDestroy(c->main_mark);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -435,23 +546,41 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
   v->res_alloc.assign(v->nb, Alloc{});
   v->free_lists.assign(16, {});
   v->rans_status.assign(v->nb, 0);
+  c->volumes.push_back(v);
   *out = v;
   return 0;
 }
 
-void fcs_volume_free(fcs_volume* v) {
-  if (!v) return;
-  if (v->ctx) (void)hipSetDevice(v->ctx->device);
+}  // extern "C"
+
+void fcs_volume_release_device(fcs_volume* v) {
+  fcs_ctx* c = v->ctx;
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)settle_prefetch(v);
+  (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab,
-                  v->d_cache, v->d_req, v->d_req_status, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel,
-                  v->d_ft, v->d_visited, v->d_misses};
+                  v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
+                  v->d_misses, v->d_csum};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  free_lane(v->lane_main);
+  free_lane(v->lane_aux);
+  auto& vs = c->volumes;
+  vs.erase(std::remove(vs.begin(), vs.end(), v), vs.end());
+  v->ctx = nullptr;
+}
+
+extern "C" {
+
+void fcs_volume_free(fcs_volume* v) {
+  if (!v) return;
+  fcs_volume_release_device(v);
   delete v;
 }
 
 int fcs_volume_get_info(const fcs_volume* v, fcs_volume_info* o) {
-  if (!v || !o) return FCS_E_INVALID;
+  if (!v || !v->ctx || !o) return FCS_E_INVALID;
   memset(o, 0, sizeof *o);
   o->num_bricks = v->nb;
   o->brick_size = v->b;
@@ -471,7 +600,7 @@ int fcs_volume_get_info(const fcs_volume* v, fcs_volume_info* o) {
 }
 
 int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
-  if (!v) return FCS_E_INVALID;
+  if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   RansArgs a{};
@@ -534,11 +663,15 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
 }
 
 int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
-  if (!v || voxels == 0) return FCS_E_INVALID;
+  if (!v || !v->ctx || voxels == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (v->d_cache) { (void)hipFree(v->d_cache); v->d_cache = nullptr; }
   v->cap = 0;
+  v->direct = false;
+  v->stride = 0;
   if (hipMalloc(&v->d_cache, voxels * v->lbytes()) != hipSuccess)
     return fail(c, FCS_E_NOMEM, "brick cache pool allocation failed");
   v->cap = voxels;
@@ -546,7 +679,8 @@ int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
 }
 
 int fcs_cache_clear(fcs_volume* v) {
-  if (!v) return FCS_E_INVALID;
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  if (int rc = settle_prefetch(v)) return rc;
   v->bump = 0;
   for (auto& f : v->free_lists) f.clear();
   std::fill(v->res_lod.begin(), v->res_lod.end(), (int16_t)-1);
@@ -554,11 +688,21 @@ int fcs_cache_clear(fcs_volume* v) {
   return 0;
 }
 
+int fcs_cache_reserve_direct(fcs_volume* v) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  const uint64_t finest = 1ull << (3 * v->L);
+  if (int rc = fcs_cache_reserve(v, finest * v->nb)) return rc;
+  v->direct = true;
+  v->stride = finest;
+  return 0;
+}
+
 int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
                        uint64_t* out_off, uint32_t* out_status) {
-  if (!v || (n && (!brick || !lod))) return FCS_E_INVALID;
+  if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
   if (!v->rans_ready) {
     int rc = fcs_decode_rans(v, nullptr);
     if (rc && rc != FCS_E_CORRUPT) return rc;
@@ -568,11 +712,24 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
   std::vector<Alloc> deferred;
   std::vector<int64_t> req_of(n, -1);
   std::vector<uint32_t> st(n, 0);
+  // direct-mapped pool: a brick holds one LOD at a time, so only its last
+  // request of the call is decoded; earlier ones share that request's slot
+  std::vector<int64_t> last_of;
+  if (v->direct) {
+    last_of.assign(v->nb, -1);
+    for (uint32_t i = 0; i < n; ++i)
+      if (brick[i] < v->nb && lod[i] <= v->L) last_of[brick[i]] = i;
+  }
+  std::vector<int64_t> alias(n, -1);
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t bi = brick[i], t = lod[i];
     if (bi >= v->nb || t > v->L) {
       st[i] = FCS_ST_BAD_REQUEST;
       if (out_off) out_off[i] = ~0ull;
+      continue;
+    }
+    if (v->direct && last_of[bi] != (int64_t)i) {
+      alias[i] = last_of[bi];
       continue;
     }
     if (v->res_lod[bi] >= 0 && ((v->res_pyr[bi] && (uint32_t)v->res_lod[bi] >= t) ||
@@ -583,7 +740,7 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
     cache_release(v, bi, &deferred);
     uint64_t off;
     const uint64_t sz = 1ull << (3 * t);
-    if (!cache_alloc(v, sz, &off)) {
+    if (!cache_alloc(v, bi, sz, &off)) {
       st[i] = FCS_ST_CACHE_FULL;
       if (out_off) out_off[i] = ~0ull;
       continue;
@@ -602,6 +759,12 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
   bool bad = false;
   for (uint32_t i = 0; i < n; ++i) {
     if (req_of[i] >= 0) st[i] |= rs[req_of[i]];
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    if (alias[i] >= 0) {
+      st[i] = st[alias[i]];
+      if (out_off) out_off[i] = out_off[alias[i]];
+    }
     bad |= st[i] != 0;
     if (out_status) out_status[i] = st[i];
   }
@@ -610,22 +773,24 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
 }
 
 int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
-  if (!v) return FCS_E_INVALID;
+  if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
   if (!v->rans_ready) {
     int rc = fcs_decode_rans(v, nullptr);
     if (rc && rc != FCS_E_CORRUPT) return rc;
   }
   const uint32_t t = std::min(lod, v->L);
   const uint64_t per = pyramid ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
-  if (!v->d_cache || v->cap < per * v->nb) {
+  if (!v->d_cache || (v->direct ? v->stride < per : v->cap < per * v->nb)) {
     if (int rc = fcs_cache_reserve(v, per * v->nb)) return rc;
   }
   fcs_cache_clear(v);
   std::vector<DecodeReq> reqs(v->nb);
+  const uint64_t slot = v->direct ? v->stride : per;
   for (uint32_t i = 0; i < v->nb; ++i) {
-    const uint64_t off = per * i;
+    const uint64_t off = slot * i;
     reqs[i] = DecodeReq{i, t | (pyramid ? FCS_REQ_PYRAMID : 0u), off};
     v->res_lod[i] = (int16_t)t;
     v->res_pyr[i] = pyramid ? 1 : 0;
@@ -652,9 +817,10 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
 }
 
 int fcs_read_cache(fcs_volume* v, uint64_t off, uint64_t count, void* host) {
-  if (!v || !host || off + count > v->cap) return FCS_E_INVALID;
+  if (!v || !v->ctx || !host || off + count > v->cap) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
   HIP_TRY(c, hipMemcpyAsync(host, (uint8_t*)v->d_cache + off * v->lbytes(), count * v->lbytes(),
                             hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -662,7 +828,7 @@ int fcs_read_cache(fcs_volume* v, uint64_t off, uint64_t count, void* host) {
 }
 
 int fcs_read_brick(fcs_volume* v, uint32_t brick, uint32_t lod, void* host) {
-  if (!v || !host || brick >= v->nb || lod > v->L) return FCS_E_INVALID;
+  if (!v || !v->ctx || !host || brick >= v->nb || lod > v->L) return FCS_E_INVALID;
   const uint8_t t = (uint8_t)lod;
   uint64_t off = 0;
   uint32_t st = 0;
@@ -672,7 +838,7 @@ int fcs_read_brick(fcs_volume* v, uint32_t brick, uint32_t lod, void* host) {
 }
 
 int fcs_brick_lods(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint32_t h, uint8_t* out) {
-  if (!v || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   // calculateBrickLOD (camera.js:103-137) with screen width min(w, h)
   // (webgpuSetup.js:587); focal length from the camera's tan(fov/2)
   const double sw = (double)std::min(w, h);
@@ -695,7 +861,7 @@ int fcs_brick_lods(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint3
 }
 
 int fcs_camera_orbit(const fcs_volume* v, double yaw, double pitch, double radius, double fov, fcs_camera* o) {
-  if (!v || !o) return FCS_E_INVALID;
+  if (!v || !v->ctx || !o) return FCS_E_INVALID;
   // OrbitController.apply (orbitController.js:100-123) around the volume
   // centre; Camera.getUniformData basis (camera.js:61-77)
   const double G = (double)v->B * v->b;
@@ -727,7 +893,7 @@ int fcs_camera_orbit(const fcs_volume* v, double yaw, double pitch, double radiu
 
 int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
                fcs_frame* out) {
-  if (!v || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   std::vector<uint8_t> lods(v->nb);
@@ -736,11 +902,15 @@ int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
   } else {
     fcs_brick_lods(v, cam, w, h, lods.data());
   }
-  // on-demand decode of every non-empty brick not resident at its LOD
+  // decodes issued ahead on the aux stream finish before this frame's work
+  if (c->aux_pending) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
+  // on-demand decode of every non-empty brick in the view frustum that is not
+  // resident at its LOD (rays never enter a brick outside the frustum)
+  const std::vector<uint8_t> vis = frustum_mask(v, cam, w, h);
   std::vector<uint32_t> qb;
   std::vector<uint8_t> ql;
   for (uint32_t i = 0; i < v->nb; ++i) {
-    if (v->empty[i] || lods[i] == 0) continue;
+    if (v->empty[i] || lods[i] == 0 || !vis[i]) continue;
     const int r = v->res_lod[i];
     const bool hit = r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= lods[i]) || (!v->res_pyr[i] && (uint32_t)r == lods[i]));
     if (!hit) { qb.push_back(i); ql.push_back(lods[i]); }
@@ -806,10 +976,116 @@ int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
   if (out->brick_visited)
     HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  v->frame_px = npx;
+  if (int rc = settle_prefetch(v)) return rc;
   float ms = 0;
   HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   out->march_ms = ms;
   c->timing.march_ms = ms;
+  return 0;
+}
+
+int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n) {
+  if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!v->rans_ready) {
+    int rc = fcs_decode_rans(v, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+  }
+  if (int rc = ensure_cache(v)) return rc;
+  if (int rc = settle_prefetch(v)) return rc;  // one prefetch batch in flight
+  std::vector<DecodeReq> reqs;
+  std::vector<Alloc> deferred;
+  std::vector<uint8_t> seen(v->nb, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t bi = brick[i], t = lod[i];
+    if (bi >= v->nb || t > v->L || t == 0 || seen[bi] || resident_at(v, bi, t)) continue;
+    seen[bi] = 1;
+    cache_release(v, bi, &deferred);
+    uint64_t off;
+    const uint64_t sz = 1ull << (3 * t);
+    if (!cache_alloc(v, bi, sz, &off)) continue;  // best effort: a full pool skips the prefetch
+    v->res_lod[bi] = (int16_t)t;
+    v->res_pyr[bi] = 0;
+    v->res_alloc[bi] = Alloc{off, sz};
+    reqs.push_back(DecodeReq{bi, t, off});
+  }
+  if (reqs.empty()) {
+    cache_free_deferred(v, deferred);
+    return 0;
+  }
+  // the aux stream runs after everything already queued on the main stream
+  HIP_TRY(c, hipEventRecord(c->main_mark, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->aux, c->main_mark, 0));
+  const int rc = launch_decode(v, v->lane_aux, reqs, c->aux, false);
+  cache_free_deferred(v, deferred);
+  if (rc) return rc;
+  HIP_TRY(c, hipEventRecord(c->aux_done, c->aux));
+  c->aux_pending = true;
+  return (int)reqs.size();
+}
+
+int fcs_frame_checksum(fcs_volume* v, uint64_t* out) {
+  if (!v || !v->ctx || !out) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  if (!v->frame_px || !v->d_flabel) return fail(c, FCS_E_INVALID, "no frame rendered yet");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!v->d_csum) HIP_TRY(c, hipMalloc((void**)&v->d_csum, 16));
+  HIP_TRY(c, fcs_launch_zero(reinterpret_cast<uint32_t*>(v->d_csum), 2, c->stream));
+  if (v->label_bits == 32)
+    HIP_TRY(c, fcs_launch_frame_checksum<uint32_t>((const uint32_t*)v->d_flabel, v->d_fvoxel, v->frame_px, v->d_csum,
+                                                   c->stream));
+  else
+    HIP_TRY(c, fcs_launch_frame_checksum<uint64_t>((const uint64_t*)v->d_flabel, v->d_fvoxel, v->frame_px, v->d_csum,
+                                                   c->stream));
+  unsigned long long h = 0;
+  HIP_TRY(c, hipMemcpyAsync(&h, v->d_csum, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *out = h;
+  return 0;
+}
+
+int fcs_camera_features(const fcs_volume* v, const fcs_camera* cam, float* out12) {
+  if (!v || !v->ctx || !cam || !out12) return FCS_E_INVALID;
+  const float g = (float)(v->B * v->b);
+  for (int k = 0; k < 3; ++k) {
+    out12[k] = cam->position[k] / g;
+    out12[3 + k] = cam->forward[k];
+    out12[6 + k] = cam->up[k];
+    out12[9 + k] = cam->right[k];
+  }
+  return 0;
+}
+
+int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead, uint32_t w,
+                        uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out, uint32_t* out_prefetched) {
+  if (!v || !v->ctx || !m || !cam || !out) return FCS_E_INVALID;
+  if (fcs_mlp_inputs(m) == 0) return fail(v->ctx, FCS_E_INVALID, "prefetch network was released");
+  if (out_prefetched) *out_prefetched = 0;
+  fcs_ctx* c = v->ctx;
+  if (fcs_mlp_outputs(m) != v->nb || fcs_mlp_inputs(m) != 12 + v->nb)
+    return fail(c, FCS_E_INVALID, "prefetch network must have 12 + bricks inputs and bricks outputs");
+  if (int rc = fcs_render(v, cam, nullptr, w, h, out)) return rc;
+  float feat[12];
+  fcs_camera_features(v, cam, feat);
+  const float* d_probs = nullptr;
+  if (int rc = fcs_mlp_predict_dev(m, feat, 12, v->d_visited, &d_probs)) return rc;
+  std::vector<float> probs(v->nb);
+  HIP_TRY(c, hipMemcpyAsync(probs.data(), d_probs, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::vector<uint8_t> lods(v->nb);
+  fcs_brick_lods(v, cam_ahead ? cam_ahead : cam, w, h, lods.data());
+  std::vector<uint32_t> cand;
+  for (uint32_t i = 0; i < v->nb; ++i)
+    if (probs[i] >= threshold && !v->empty[i] && lods[i] > 0 && !resident_at(v, i, lods[i])) cand.push_back(i);
+  std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return probs[a] > probs[b]; });
+  if (cand.size() > max_prefetch) cand.resize(max_prefetch);
+  std::vector<uint8_t> cl(cand.size());
+  for (size_t i = 0; i < cand.size(); ++i) cl[i] = lods[cand[i]];
+  const int n = fcs_prefetch_bricks(v, cand.data(), cl.data(), (uint32_t)cand.size());
+  if (n < 0) return n;
+  if (out_prefetched) *out_prefetched = (uint32_t)n;
   return 0;
 }
 

@@ -1,0 +1,56 @@
+// Host-side internals shared by the runtime's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fcs/fcs.h"
+
+struct fcs_volume;
+struct fcs_mlp;
+
+// One HIP device and its streams (fcs_create). `stream` runs requests and
+// frames; `aux` runs work issued ahead of them (prefetch decodes). The context
+// tracks the volumes and networks created on it: fcs_destroy releases their
+// device memory and detaches them, so freeing them afterwards is safe.
+struct fcs_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t aux_done = nullptr;   // recorded on `aux` after the last prefetch launch
+  hipEvent_t main_mark = nullptr;  // recorded on `stream` before a prefetch launch
+  bool aux_pending = false;
+  std::string err;
+  fcs_timing timing{};
+  int rans_mapping = FCS_RANS_MAPPING_AUTO;
+  std::vector<fcs_volume*> volumes;
+  std::vector<fcs_mlp*> mlps;
+};
+
+// Release a volume's / network's device memory and detach it from its
+// context (called by fcs_destroy for the ones still alive).
+void fcs_volume_release_device(fcs_volume* v);
+void fcs_mlp_release_device(fcs_mlp* m);
+
+inline int fcs_fail(fcs_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define FCS_HIP_TRY(ctx, expr)                                                            \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fcs_fail((ctx), FCS_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Learned prefetch network (fcs_mlp.hip): probabilities of one sample, from a
+// pose and a device mask of one u32 per input position, in a device buffer
+// owned by the network (valid until its next call).
+struct fcs_mlp;
+int fcs_mlp_predict_dev(fcs_mlp* m, const float* pose, uint32_t n_pose, const uint32_t* d_mask, const float** d_probs);
+uint32_t fcs_mlp_outputs(const fcs_mlp* m);
+uint32_t fcs_mlp_inputs(const fcs_mlp* m);

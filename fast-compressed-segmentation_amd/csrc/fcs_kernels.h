@@ -79,3 +79,6 @@ hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, 
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s);
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s);
+template <typename LT>
+hipError_t fcs_launch_frame_checksum(const LT* label, const uint64_t* voxel, uint64_t n, unsigned long long* out,
+                                     hipStream_t s);

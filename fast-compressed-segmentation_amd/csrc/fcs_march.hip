@@ -228,5 +228,46 @@ hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Order-independent 64-bit checksum of a frame: the sum over pixels of a
+// mixed (label, voxel index, pixel index) word, so equal frames give equal
+// sums and any changed pixel changes it with overwhelming probability.
+namespace {
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename LT>
+__global__ void frame_checksum_kernel(const LT* __restrict__ label, const uint64_t* __restrict__ voxel, uint64_t n,
+                                      unsigned long long* out) {
+  __shared__ unsigned long long red[256];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long h = 0;
+  if (i < n) h = mix64(mix64((uint64_t)label[i] + 0x9E3779B97F4A7C15ull * (i + 1)) ^ voxel[i]);
+  red[threadIdx.x] = h;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+}  // namespace
+
+template <typename LT>
+hipError_t fcs_launch_frame_checksum(const LT* label, const uint64_t* voxel, uint64_t n, unsigned long long* out,
+                                     hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(frame_checksum_kernel<LT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, label, voxel, n,
+                     out);
+  return hipGetLastError();
+}
+
+template hipError_t fcs_launch_frame_checksum<uint32_t>(const uint32_t*, const uint64_t*, uint64_t,
+                                                        unsigned long long*, hipStream_t);
+template hipError_t fcs_launch_frame_checksum<uint64_t>(const uint64_t*, const uint64_t*, uint64_t,
+                                                        unsigned long long*, hipStream_t);
+
 template hipError_t fcs_launch_march<uint32_t>(const MarchArgs<uint32_t>&, hipStream_t);
 template hipError_t fcs_launch_march<uint64_t>(const MarchArgs<uint64_t>&, hipStream_t);

@@ -148,6 +148,7 @@ _sig("fcs_volume_get_info", ctypes.c_int, c_vp, ctypes.POINTER(VolumeInfo))
 _sig("fcs_decode_rans", ctypes.c_int, c_vp, c_u32p)
 _sig("fcs_cache_reserve", ctypes.c_int, c_vp, ctypes.c_uint64)
 _sig("fcs_cache_clear", ctypes.c_int, c_vp)
+_sig("fcs_cache_reserve_direct", ctypes.c_int, c_vp)
 _sig("fcs_request_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32, c_u64p, c_u32p)
 _sig("fcs_decode_all", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_int)
 _sig("fcs_read_brick", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, c_vp)
@@ -157,6 +158,12 @@ _sig("fcs_render", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), c_u8p, ctyp
      ctypes.POINTER(FrameStruct))
 _sig("fcs_camera_orbit", ctypes.c_int, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
      ctypes.POINTER(CameraStruct))
+_sig("fcs_prefetch_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32)
+_sig("fcs_frame_checksum", ctypes.c_int, c_vp, c_u64p)
+_sig("fcs_camera_features", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), ctypes.POINTER(ctypes.c_float))
+_sig("fcs_render_prefetch", ctypes.c_int, c_vp, c_vp, ctypes.POINTER(CameraStruct), ctypes.POINTER(CameraStruct),
+     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(FrameStruct),
+     ctypes.POINTER(ctypes.c_uint32))
 _sig("fcs_gen_preset", ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(GenParams))
 _sig("fcs_gen_create", ctypes.c_int, ctypes.POINTER(GenParams), ctypes.POINTER(c_vp))
 _sig("fcs_gen_destroy", None, c_vp)

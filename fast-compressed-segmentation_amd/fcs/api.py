@@ -143,6 +143,10 @@ class Volume:
     def cache_reserve(self, voxels):
         check(lib.fcs_cache_reserve(self.h, int(voxels)), self.ctx.h)
 
+    def cache_reserve_direct(self):
+        """Direct-mapped pool: brick i owns slot i (fcs_cache_reserve_direct)."""
+        check(lib.fcs_cache_reserve_direct(self.h), self.ctx.h)
+
     def cache_clear(self):
         check(lib.fcs_cache_clear(self.h), self.ctx.h)
 
@@ -179,6 +183,55 @@ class Volume:
         cam = Camera((0, 0, 0), (0, 0, 1), (1, 0, 0), (0, 1, 0))
         check(lib.fcs_camera_orbit(self.h, yaw, pitch, radius, fov, ctypes.byref(cam.s)), self.ctx.h)
         return cam
+
+    def prefetch(self, bricks, lods):
+        """Decode (brick, LOD) pairs ahead, asynchronously (fcs_prefetch_bricks)."""
+        b = np.ascontiguousarray(bricks, dtype=np.uint32)
+        l = np.ascontiguousarray(lods, dtype=np.uint8)
+        return check(lib.fcs_prefetch_bricks(self.h, _ptr(b, c_u32p), _ptr(l, c_u8p), b.size), self.ctx.h)
+
+    def frame_checksum(self):
+        out = ctypes.c_uint64()
+        check(lib.fcs_frame_checksum(self.h, ctypes.byref(out)), self.ctx.h)
+        return out.value
+
+    def camera_features(self, cam):
+        out = (ctypes.c_float * 12)()
+        check(lib.fcs_camera_features(self.h, ctypes.byref(cam.s), out), self.ctx.h)
+        return np.array(out[:], np.float32)
+
+    def render_prefetch(self, mlp, cam, w, h, cam_ahead=None, threshold=0.5, max_prefetch=4096,
+                        want=("brick_visited",)):
+        """One frame with learned prefetch (fcs_render_prefetch); returns (Frame, decodes issued)."""
+        fr, f = self._frame_buffers(w, h, want)
+        n = ctypes.c_uint32()
+        ca = ctypes.byref(cam_ahead.s) if cam_ahead is not None else None
+        check(lib.fcs_render_prefetch(self.h, mlp.h, ctypes.byref(cam.s), ca, w, h, threshold, max_prefetch,
+                                      ctypes.byref(f), ctypes.byref(n)), self.ctx.h)
+        return self._frame(fr, f), n.value
+
+    def _frame_buffers(self, w, h, want):
+        n = w * h
+        nb = self.info["num_bricks"]
+        fr = {
+            "label": np.zeros(n, self.label_dtype) if "label" in want else None,
+            "voxel": np.zeros(n, np.uint64) if "voxel" in want else None,
+            "t_hit": np.zeros(n, np.float32) if "t_hit" in want else None,
+            "vis": np.zeros(nb, np.uint32) if "brick_visited" in want else None,
+            "used": np.zeros(nb, np.uint8),
+        }
+        f = FrameStruct()
+        f.label = fr["label"].ctypes.data if fr["label"] is not None else None
+        f.voxel = _ptr(fr["voxel"], c_u64p) if fr["voxel"] is not None else None
+        f.t_hit = fr["t_hit"].ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fr["t_hit"] is not None else None
+        f.brick_visited = _ptr(fr["vis"], c_u32p) if fr["vis"] is not None else None
+        f.brick_lod_used = _ptr(fr["used"], c_u8p)
+        return fr, f
+
+    @staticmethod
+    def _frame(fr, f):
+        return Frame(fr["label"], fr["voxel"], fr["t_hit"], fr["vis"], fr["used"], f.bricks_decoded, f.decode_ms,
+                     f.march_ms)
 
     def render(self, cam, w, h, brick_lod=None, want=("label", "voxel", "t_hit", "brick_visited")):
         n = w * h

@@ -176,11 +176,41 @@ int fcs_read_cache(fcs_volume* vol, uint64_t offset, uint64_t count, void* host_
 int fcs_brick_lods(const fcs_volume* vol, const fcs_camera* cam, uint32_t w, uint32_t h,
                    uint8_t* out_lods);
 
-/* Render one frame: brick_lod (nullable: LOD rule) selects each brick's LOD;
- * non-resident bricks at that LOD are decoded first (on demand), then every
- * pixel is marched. */
+/* No reference counterpart (the reference's pool is a bump allocator with
+ * per-LOD free stacks, cacheManager.js:52-78, as fcs_cache_reserve's is):
+ * replace the pool with a direct-mapped one of bricks x 8^L voxels in which
+ * brick i owns slot i, so regions never move or fragment. A brick then holds
+ * one LOD at a time: when one fcs_request_bricks call asks for a brick more
+ * than once, only its last request is decoded and the earlier ones report
+ * that request's slot and status. */
+int fcs_cache_reserve_direct(fcs_volume* vol);
+
+/* Render one frame (replaces renderLoop.js:268-298, the frame's decode pass +
+ * rayTrace pass): brick_lod (nullable: LOD rule) selects each brick's LOD;
+ * non-empty bricks inside the view frustum (culling.js:87-171, exact
+ * perspective planes) that are not resident at that LOD are decoded first,
+ * on demand, then every pixel is marched. Decodes issued ahead with
+ * fcs_prefetch_bricks finish before the frame's own work. Output arrays left
+ * NULL in `out` are not copied back. */
 int fcs_render(fcs_volume* vol, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w,
                uint32_t h, fcs_frame* out);
+
+/* No reference counterpart (the reference only decodes on demand): decode
+ * (brick, LOD) pairs ahead of the frames that need them, asynchronously on the
+ * context's second stream. Residency is updated at once; the next fcs_render
+ * (or any cache call) orders itself after these decodes. A prefetch that fails
+ * leaves its brick non-resident, so on-demand decode retries and reports it.
+ * Returns the number of decodes issued (pairs already resident are skipped). */
+int fcs_prefetch_bricks(fcs_volume* vol, const uint32_t* brick, const uint8_t* lod, uint32_t n);
+
+/* No reference counterpart: order-independent 64-bit checksum of the last
+ * rendered frame's first-hit labels and voxel indices (computed on the
+ * device), for comparing frames without copying them back. */
+int fcs_frame_checksum(fcs_volume* vol, uint64_t* out);
+
+/* No reference counterpart: the 12 camera features the prefetch network reads,
+ * position / grid size, forward, up, right. */
+int fcs_camera_features(const fcs_volume* vol, const fcs_camera* cam, float* out12);
 
 /* Orbit pose around the volume centre (orbitController.js:100-123) with the
  * basis of camera.js:61-77, fov in radians. */
