@@ -41,9 +41,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3),
-                    help="2: 512^3 demo scale (default); 3: 2048^3 full decode + 100k batched requests")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4),
+                    help="2: 512^3 demo scale (default); 3: 2048^3 full decode + 100k batched requests; "
+                         "4: 1024^3 fly-throughs, learned prefetch vs on-demand")
     ap.add_argument("--size", type=int, default=0, help="override the volume edge (config 3)")
+    ap.add_argument("--train-paths", type=int, default=20)
+    ap.add_argument("--eval-paths", type=int, default=5)
+    ap.add_argument("--frames", type=int, default=600)
+    ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--every", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU sharding is "
@@ -56,6 +63,8 @@ def main():
 
     if args.config == 3:
         return bench_config3(args)
+    if args.config == 4:
+        return bench_config4(args)
 
     t0 = time.time()
     gen = fcs.Generator(2)
@@ -261,6 +270,88 @@ def bench_config3(args):
     }
     print(json.dumps(out))
 
+
+def bench_config4(args):
+    """1024^3 u32 volume (seed 3, the 512^3 recipe at the same seed density),
+    32^3 bricks (32,768), direct-mapped 4 GiB brick cache. Camera paths are
+    Catmull-Rom splines through 8 seeded random control points inside the
+    volume, 600 frames at 1920x1080 each. The plain marcher's visit logs of
+    `train_paths` paths train the prefetch MLP (12 pose features + 32^3
+    visited mask -> 512 -> 512 -> 32^3, FP32, on the GPU); `eval_paths`
+    held-out paths are then rendered from a cold cache three ways: plain
+    on-demand (fcs_render: decode the frustum's non-resident bricks, then
+    march), visibility-driven on-demand (fcs_render_visible) and learned
+    prefetch (the network every `every` frames, fcs_render_prefetch).
+    value = 1080p frames/s of learned prefetch over the held-out paths (wall
+    clock per frame, host work included); every frame's checksum is compared
+    with the plain form's."""
+    import numpy as np
+
+    import fcs
+    from fcs import paths
+    from fcs import prefetch as PF
+
+    W, H = 1920, 1080
+    t0 = time.time()
+    gen = fcs.Generator(4)
+    data = fcs.encode_generated(gen, 32)
+    log(f"[bench] generated + encoded {gen.size}^3 volume: {len(data)} bytes in {time.time() - t0:.1f}s")
+    ctx = fcs.Context(0)
+    vol = ctx.load(data)
+    if np.any(vol.decode_rans()):
+        raise RuntimeError("corrupt bricks")
+    vol.cache_reserve_direct()
+    nb, G = vol.info["num_bricks"], float(vol.info["grid_size"])
+    horizon = 3 * args.every
+    samples = []
+    t1 = time.time()
+    for p in range(args.train_paths):
+        feats, vis = PF.collect_visits(vol, paths.flythrough(3, p, G, frames=args.frames), W, H)
+        samples.append(PF.build_samples(feats, vis, horizon))
+        log(f"[bench] visit log of training path {p}: {vis.sum(1).mean():.0f} bricks visited per frame")
+    collect_s = time.time() - t1
+    mlp, losses, train_s, n_samples = PF.train(ctx, nb, samples, hidden=512, epochs=args.epochs, batch=64, lr=1e-3)
+    log(f"[bench] trained on {n_samples} samples in {train_s:.2f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    runs = {"plain": [], "visible": [], "learned": []}
+    for p in range(args.train_paths, args.train_paths + args.eval_paths):
+        cams = paths.flythrough(3, p, G, frames=args.frames)
+        runs["plain"].append(PF.run_path(vol, cams, W, H, mode="plain"))
+        runs["visible"].append(PF.run_path(vol, cams, W, H, mode="visible"))
+        runs["learned"].append(PF.run_path(vol, cams, W, H, mlp=mlp, threshold=args.threshold, max_prefetch=4096,
+                                           every=args.every, ahead=args.every))
+        log(f"[bench] held-out path {p} done")
+    learned = PF.compare(runs["plain"], runs["learned"])
+    visible = PF.compare(runs["plain"], runs["visible"])
+    frames = learned["frames"]
+    out = {
+        "metric": "1080p frames/s, learned prefetch vs plain on-demand GPU form",
+        "value": learned["fps_prefetch"],
+        "unit": "frames/s",
+        "n_gpus": 1, "steps": frames, "warmup": 0,
+        "ms_per_step": 1000.0 / learned["fps_prefetch"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 labels, fp32 network",
+        "data": "synthetic",
+        "config": {"workload": f"{gen.size}^3 u32 Voronoi (seed 3), 32^3 bricks x{nb}, direct-mapped 4 GiB cache; "
+                               f"{args.train_paths} training + {args.eval_paths} held-out Catmull-Rom paths x "
+                               f"{args.frames} frames at 1920x1080",
+                   "network": "12 + 32768 -> 512 -> 512 -> 32768 FP32",
+                   "threshold": args.threshold, "every": args.every, "horizon": horizon},
+        "fps_plain_on_demand": learned["fps_plain"],
+        "fps_visibility_driven": visible["fps_prefetch"],
+        "fps_learned_prefetch": learned["fps_prefetch"],
+        "speedup_vs_plain": learned["speedup"],
+        "speedup_visibility_driven_vs_plain": visible["speedup"],
+        "miss_rate": learned["miss_rate"],
+        "miss_rate_visibility_driven": visible["miss_rate"],
+        "on_demand_decodes": {"plain": learned["on_demand_plain"], "visibility_driven": visible["on_demand_prefetch"],
+                              "learned": learned["on_demand_prefetch"]},
+        "prefetched": learned["prefetched"], "wasted_prefetches": learned["wasted"],
+        "wasted_frac": learned["wasted_frac"],
+        "label_mismatch_frames": learned["label_mismatch_frames"] + visible["label_mismatch_frames"],
+        "train": {"samples": n_samples, "epochs": args.epochs, "seconds": train_s, "loss_first": float(losses[0]),
+                  "loss_last": float(losses[-1]), "visit_log_seconds": collect_s},
+    }
+    print(json.dumps(out))
 
 if __name__ == "__main__":
     main()

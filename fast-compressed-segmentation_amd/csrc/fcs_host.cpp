@@ -293,6 +293,37 @@ int settle_prefetch(fcs_volume* v) {
   return 0;
 }
 
+// calculateBrickLOD (camera.js:103-137) for one brick: the largest LOD t
+// with focal * (b / 2^t) / d >= 1, d = distance from the camera to the brick
+// centre, focal from the camera's tan(fov/2) and screen width min(w, h)
+// (webgpuSetup.js:587). float64 like the reference's JS numbers.
+struct LodRule {
+  const fcs_volume* v;
+  double px, py, pz, focal, half;
+  double voxel[32];
+  LodRule(const fcs_volume* vol, const fcs_camera* cam, uint32_t w, uint32_t h) : v(vol) {
+    const double sw = (double)std::min(w, h);
+    focal = (sw / 2.0) / (double)cam->tan_half_fov;
+    half = v->b / 2.0;
+    px = cam->position[0];
+    py = cam->position[1];
+    pz = cam->position[2];
+    for (uint32_t l = 0; l <= v->L; ++l) voxel[l] = (double)v->b / std::pow(2.0, (double)l);  // exact powers of two
+  }
+  uint8_t operator()(uint32_t i) const {
+    const double cx = fcs_unmorton_x(i) * (double)v->b + half, cy = fcs_unmorton_y(i) * (double)v->b + half,
+                 cz = fcs_unmorton_z(i) * (double)v->b + half;
+    const double dx = cx - px, dy = cy - py, dz = cz - pz;
+    double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (d < 0.1) d = 0.1;
+    uint32_t best = 0;
+    for (uint32_t l = 0; l <= v->L; ++l) {
+      if ((focal * voxel[l]) / d >= 1.0) best = l; else break;
+    }
+    return (uint8_t)best;
+  }
+};
+
 // Bricks a ray of the frame can enter. Perspective: the brick box (grown by
 // one voxel against rounding) is tested against the four side planes through
 // the camera position and the plane behind it; a box wholly outside one plane
@@ -841,22 +872,8 @@ int fcs_brick_lods(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint3
   if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   // calculateBrickLOD (camera.js:103-137) with screen width min(w, h)
   // (webgpuSetup.js:587); focal length from the camera's tan(fov/2)
-  const double sw = (double)std::min(w, h);
-  const double focal = (sw / 2.0) / (double)cam->tan_half_fov;
-  const double half = v->b / 2.0;
-  for (uint32_t i = 0; i < v->nb; ++i) {
-    const double cx = fcs_unmorton_x(i) * (double)v->b + half, cy = fcs_unmorton_y(i) * (double)v->b + half,
-                 cz = fcs_unmorton_z(i) * (double)v->b + half;
-    const double dx = cx - cam->position[0], dy = cy - cam->position[1], dz = cz - cam->position[2];
-    double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-    if (d < 0.1) d = 0.1;
-    uint32_t best = 0;
-    for (uint32_t l = 0; l <= v->L; ++l) {
-      const double voxel = (double)v->b / std::pow(2.0, (double)l);
-      if ((focal * voxel) / d >= 1.0) best = l; else break;
-    }
-    out[i] = (uint8_t)best;
-  }
+  const LodRule rule(v, cam, w, h);
+  for (uint32_t i = 0; i < v->nb; ++i) out[i] = rule(i);
   return 0;
 }
 
@@ -891,56 +908,44 @@ int fcs_camera_orbit(const fcs_volume* v, double yaw, double pitch, double radiu
   return 0;
 }
 
-int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
-               fcs_frame* out) {
-  if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
-  fcs_ctx* c = v->ctx;
-  HIP_TRY(c, hipSetDevice(c->device));
-  std::vector<uint8_t> lods(v->nb);
+}  // extern "C"
+
+namespace {
+
+int frame_lods(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+               std::vector<uint8_t>& lods) {
+  lods.resize(v->nb);
   if (brick_lod) {
     for (uint32_t i = 0; i < v->nb; ++i) lods[i] = std::min<uint8_t>(brick_lod[i], (uint8_t)v->L);
-  } else {
-    fcs_brick_lods(v, cam, w, h, lods.data());
+    return 0;
   }
-  // decodes issued ahead on the aux stream finish before this frame's work
-  if (c->aux_pending) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
-  // on-demand decode of every non-empty brick in the view frustum that is not
-  // resident at its LOD (rays never enter a brick outside the frustum)
-  const std::vector<uint8_t> vis = frustum_mask(v, cam, w, h);
-  std::vector<uint32_t> qb;
-  std::vector<uint8_t> ql;
-  for (uint32_t i = 0; i < v->nb; ++i) {
-    if (v->empty[i] || lods[i] == 0 || !vis[i]) continue;
-    const int r = v->res_lod[i];
-    const bool hit = r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= lods[i]) || (!v->res_pyr[i] && (uint32_t)r == lods[i]));
-    if (!hit) { qb.push_back(i); ql.push_back(lods[i]); }
-  }
-  out->bricks_decoded = (uint32_t)qb.size();
-  out->decode_ms = 0.0f;
-  if (!qb.empty()) {
-    int rc = fcs_request_bricks(v, qb.data(), ql.data(), (uint32_t)qb.size(), nullptr, nullptr);
-    if (rc && rc != FCS_E_CORRUPT) return rc;
-    out->decode_ms = c->timing.decode_ms;
-  }
-  // per-brick frame metadata
+  return fcs_brick_lods(v, cam, w, h, lods.data());
+}
+
+// Upload the frame's per-brick metadata (LOD, empty, resident at that LOD,
+// cache offset) and march every pixel on the context stream (not waited for).
+int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h) {
+  fcs_ctx* c = v->ctx;
   std::vector<uint32_t> meta(v->nb);
   std::vector<uint64_t> offs(v->nb, 0);
   for (uint32_t i = 0; i < v->nb; ++i) {
     uint32_t m = lods[i];
     if (v->empty[i]) m |= FCS_META_EMPTY;
-    const int r = v->res_lod[i];
-    if (r >= 0 && ((v->res_pyr[i] && (uint32_t)r >= lods[i]) || (!v->res_pyr[i] && (uint32_t)r == lods[i]))) {
+    if (resident_at(v, i, lods[i])) {
       m |= FCS_META_RESIDENT;
       offs[i] = v->res_alloc[i].off + (v->res_pyr[i] ? fcs_pyramid_level_offset(lods[i]) : 0);
     }
     meta[i] = m;
-    if (out->brick_lod_used) out->brick_lod_used[i] = lods[i];
   }
   const size_t npx = (size_t)w * h;
   if (npx > v->frame_cap) {
     void* ptrs[] = {v->d_flabel, v->d_fvoxel, v->d_ft};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
+    v->d_flabel = nullptr;
+    v->d_fvoxel = nullptr;
+    v->d_ft = nullptr;
+    v->frame_cap = 0;
     HIP_TRY(c, hipMalloc(&v->d_flabel, npx * v->lbytes()));
     HIP_TRY(c, hipMalloc((void**)&v->d_fvoxel, npx * 8));
     HIP_TRY(c, hipMalloc((void**)&v->d_ft, npx * 4));
@@ -970,19 +975,106 @@ int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
     HIP_TRY(c, fcs_launch_march<uint64_t>(a, c->stream));
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  return 0;
+}
+
+// Copy the requested outputs of the last march pass back, wait, and settle.
+int finish_frame(fcs_volume* v, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h, fcs_frame* out) {
+  fcs_ctx* c = v->ctx;
+  const size_t npx = (size_t)w * h;
   if (out->label) HIP_TRY(c, hipMemcpyAsync(out->label, v->d_flabel, npx * v->lbytes(), hipMemcpyDeviceToHost, c->stream));
   if (out->voxel) HIP_TRY(c, hipMemcpyAsync(out->voxel, v->d_fvoxel, npx * 8, hipMemcpyDeviceToHost, c->stream));
   if (out->t_hit) HIP_TRY(c, hipMemcpyAsync(out->t_hit, v->d_ft, npx * 4, hipMemcpyDeviceToHost, c->stream));
   if (out->brick_visited)
     HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out->brick_lod_used) memcpy(out->brick_lod_used, lods.data(), v->nb);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   v->frame_px = npx;
   if (int rc = settle_prefetch(v)) return rc;
   float ms = 0;
   HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  out->march_ms = ms;
-  c->timing.march_ms = ms;
+  out->march_ms += ms;
+  c->timing.march_ms = out->march_ms;
   return 0;
+}
+
+int decode_list(fcs_volume* v, const std::vector<uint32_t>& qb, const std::vector<uint8_t>& ql, fcs_frame* out) {
+  if (qb.empty()) return 0;
+  int rc = fcs_request_bricks(v, qb.data(), ql.data(), (uint32_t)qb.size(), nullptr, nullptr);
+  if (rc && rc != FCS_E_CORRUPT) return rc;
+  out->bricks_decoded += (uint32_t)qb.size();
+  out->decode_ms += v->ctx->timing.decode_ms;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+               fcs_frame* out) {
+  if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<uint8_t> lods;
+  if (int rc = frame_lods(v, cam, brick_lod, w, h, lods)) return rc;
+  // decodes issued ahead on the aux stream finish before this frame's work
+  if (c->aux_pending) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
+  // on-demand decode of every non-empty brick in the view frustum that is not
+  // resident at its LOD (rays never enter a brick outside the frustum)
+  const std::vector<uint8_t> vis = frustum_mask(v, cam, w, h);
+  std::vector<uint32_t> qb;
+  std::vector<uint8_t> ql;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    if (v->empty[i] || lods[i] == 0 || !vis[i] || resident_at(v, i, lods[i])) continue;
+    qb.push_back(i);
+    ql.push_back(lods[i]);
+  }
+  out->bricks_decoded = 0;
+  out->decode_ms = 0.0f;
+  out->march_ms = 0.0f;
+  if (int rc = decode_list(v, qb, ql, out)) return rc;
+  if (int rc = march_pass(v, cam, lods, w, h)) return rc;
+  return finish_frame(v, lods, w, h, out);
+}
+
+int fcs_render_visible(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+                       fcs_frame* out) {
+  if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<uint8_t> lods;
+  if (int rc = frame_lods(v, cam, brick_lod, w, h, lods)) return rc;
+  if (c->aux_pending) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
+  out->bricks_decoded = 0;
+  out->decode_ms = 0.0f;
+  out->march_ms = 0.0f;
+  std::vector<uint32_t> visited(v->nb);
+  // march with what is resident; a ray that enters a non-resident brick is
+  // counted as a miss and passes through it: decode the entered bricks that
+  // were missing and march again until no ray misses. The last pass sees
+  // every brick its rays enter, so the frame equals fcs_render's.
+  for (int pass = 0;; ++pass) {
+    if (int rc = march_pass(v, cam, lods, w, h)) return rc;
+    uint32_t misses = 0;
+    HIP_TRY(c, hipMemcpyAsync(&misses, v->d_misses, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (misses == 0) break;
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    out->march_ms += ms;
+    HIP_TRY(c, hipMemcpy(visited.data(), v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> qb;
+    std::vector<uint8_t> ql;
+    for (uint32_t i = 0; i < v->nb; ++i) {
+      if (!visited[i] || v->empty[i] || lods[i] == 0 || resident_at(v, i, lods[i])) continue;
+      qb.push_back(i);
+      ql.push_back(lods[i]);
+    }
+    if (qb.empty() || pass >= 64) return fail(c, FCS_E_CORRUPT, "misses without a decodable brick");
+    if (int rc = decode_list(v, qb, ql, out)) return rc;
+  }
+  return finish_frame(v, lods, w, h, out);
 }
 
 int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n) {
@@ -1059,14 +1151,15 @@ int fcs_camera_features(const fcs_volume* v, const fcs_camera* cam, float* out12
 }
 
 int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead, uint32_t w,
-                        uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out, uint32_t* out_prefetched) {
+                        uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out, uint32_t* out_prefetched,
+                        uint32_t* out_bricks) {
   if (!v || !v->ctx || !m || !cam || !out) return FCS_E_INVALID;
   if (fcs_mlp_inputs(m) == 0) return fail(v->ctx, FCS_E_INVALID, "prefetch network was released");
   if (out_prefetched) *out_prefetched = 0;
   fcs_ctx* c = v->ctx;
   if (fcs_mlp_outputs(m) != v->nb || fcs_mlp_inputs(m) != 12 + v->nb)
     return fail(c, FCS_E_INVALID, "prefetch network must have 12 + bricks inputs and bricks outputs");
-  if (int rc = fcs_render(v, cam, nullptr, w, h, out)) return rc;
+  if (int rc = fcs_render_visible(v, cam, nullptr, w, h, out)) return rc;
   float feat[12];
   fcs_camera_features(v, cam, feat);
   const float* d_probs = nullptr;
@@ -1074,18 +1167,26 @@ int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const 
   std::vector<float> probs(v->nb);
   HIP_TRY(c, hipMemcpyAsync(probs.data(), d_probs, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  std::vector<uint8_t> lods(v->nb);
-  fcs_brick_lods(v, cam_ahead ? cam_ahead : cam, w, h, lods.data());
+  const LodRule rule(v, cam_ahead ? cam_ahead : cam, w, h);
   std::vector<uint32_t> cand;
-  for (uint32_t i = 0; i < v->nb; ++i)
-    if (probs[i] >= threshold && !v->empty[i] && lods[i] > 0 && !resident_at(v, i, lods[i])) cand.push_back(i);
+  std::vector<uint8_t> lod_of(v->nb, 0);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    if (!(probs[i] >= threshold) || v->empty[i]) continue;
+    const uint8_t t = rule(i);
+    if (t == 0 || resident_at(v, i, t)) continue;
+    lod_of[i] = t;
+    cand.push_back(i);
+  }
   std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return probs[a] > probs[b]; });
   if (cand.size() > max_prefetch) cand.resize(max_prefetch);
   std::vector<uint8_t> cl(cand.size());
-  for (size_t i = 0; i < cand.size(); ++i) cl[i] = lods[cand[i]];
+  for (size_t i = 0; i < cand.size(); ++i) cl[i] = lod_of[cand[i]];
   const int n = fcs_prefetch_bricks(v, cand.data(), cl.data(), (uint32_t)cand.size());
   if (n < 0) return n;
   if (out_prefetched) *out_prefetched = (uint32_t)n;
+  // candidates were neither resident nor repeated, so each was issued unless the pool was full
+  if (out_bricks)
+    for (size_t i = 0; i < cand.size(); ++i) out_bricks[i] = resident_at(v, cand[i], cl[i]) ? cand[i] : ~0u;
   return 0;
 }
 

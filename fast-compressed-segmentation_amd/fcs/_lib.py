@@ -158,12 +158,14 @@ _sig("fcs_render", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), c_u8p, ctyp
      ctypes.POINTER(FrameStruct))
 _sig("fcs_camera_orbit", ctypes.c_int, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
      ctypes.POINTER(CameraStruct))
+_sig("fcs_render_visible", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), c_u8p, ctypes.c_uint32, ctypes.c_uint32,
+     ctypes.POINTER(FrameStruct))
 _sig("fcs_prefetch_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32)
 _sig("fcs_frame_checksum", ctypes.c_int, c_vp, c_u64p)
 _sig("fcs_camera_features", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), ctypes.POINTER(ctypes.c_float))
 _sig("fcs_render_prefetch", ctypes.c_int, c_vp, c_vp, ctypes.POINTER(CameraStruct), ctypes.POINTER(CameraStruct),
      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(FrameStruct),
-     ctypes.POINTER(ctypes.c_uint32))
+     ctypes.POINTER(ctypes.c_uint32), c_u32p)
 _sig("fcs_gen_preset", ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(GenParams))
 _sig("fcs_gen_create", ctypes.c_int, ctypes.POINTER(GenParams), ctypes.POINTER(c_vp))
 _sig("fcs_gen_destroy", None, c_vp)

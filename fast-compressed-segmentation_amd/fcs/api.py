@@ -184,6 +184,16 @@ class Volume:
         check(lib.fcs_camera_orbit(self.h, yaw, pitch, radius, fov, ctypes.byref(cam.s)), self.ctx.h)
         return cam
 
+    def render_visible(self, cam, w, h, brick_lod=None, want=("brick_visited",)):
+        """Frame decoding only the bricks its rays enter (fcs_render_visible)."""
+        fr, f = self._frame_buffers(w, h, want)
+        lp = None
+        if brick_lod is not None:
+            bl = np.ascontiguousarray(brick_lod, dtype=np.uint8)
+            lp = _ptr(bl, c_u8p)
+        check(lib.fcs_render_visible(self.h, ctypes.byref(cam.s), lp, w, h, ctypes.byref(f)), self.ctx.h)
+        return self._frame(fr, f)
+
     def prefetch(self, bricks, lods):
         """Decode (brick, LOD) pairs ahead, asynchronously (fcs_prefetch_bricks)."""
         b = np.ascontiguousarray(bricks, dtype=np.uint32)
@@ -202,13 +212,14 @@ class Volume:
 
     def render_prefetch(self, mlp, cam, w, h, cam_ahead=None, threshold=0.5, max_prefetch=4096,
                         want=("brick_visited",)):
-        """One frame with learned prefetch (fcs_render_prefetch); returns (Frame, decodes issued)."""
+        """One frame with learned prefetch (fcs_render_prefetch); returns (Frame, bricks prefetched)."""
         fr, f = self._frame_buffers(w, h, want)
         n = ctypes.c_uint32()
+        lst = np.full(max(1, max_prefetch), 0xFFFFFFFF, np.uint32)
         ca = ctypes.byref(cam_ahead.s) if cam_ahead is not None else None
         check(lib.fcs_render_prefetch(self.h, mlp.h, ctypes.byref(cam.s), ca, w, h, threshold, max_prefetch,
-                                      ctypes.byref(f), ctypes.byref(n)), self.ctx.h)
-        return self._frame(fr, f), n.value
+                                      ctypes.byref(f), ctypes.byref(n), _ptr(lst, c_u32p)), self.ctx.h)
+        return self._frame(fr, f), lst[lst != 0xFFFFFFFF]
 
     def _frame_buffers(self, w, h, want):
         n = w * h

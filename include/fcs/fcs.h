@@ -195,6 +195,14 @@ int fcs_cache_reserve_direct(fcs_volume* vol);
 int fcs_render(fcs_volume* vol, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w,
                uint32_t h, fcs_frame* out);
 
+/* No reference counterpart: render one frame decoding only the bricks its rays
+ * enter. March with what is resident; a ray entering a non-empty brick that is
+ * not resident at its LOD is counted and passes through it; those bricks are
+ * decoded and the frame is marched again until no ray misses, so the frame
+ * equals fcs_render's. `out->bricks_decoded` counts the decodes. */
+int fcs_render_visible(fcs_volume* vol, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w,
+                       uint32_t h, fcs_frame* out);
+
 /* No reference counterpart (the reference only decodes on demand): decode
  * (brick, LOD) pairs ahead of the frames that need them, asynchronously on the
  * context's second stream. Residency is updated at once; the next fcs_render

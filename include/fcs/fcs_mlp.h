@@ -65,16 +65,19 @@ int fcs_mlp_train_masks(fcs_mlp* m, const float* poses, uint32_t n_pose, const u
 int fcs_mlp_predict_device_mask(fcs_mlp* m, const float* pose, uint32_t n_pose, const uint32_t* d_mask_u32,
                                 float* probs);
 
-/* One frame with learned prefetch: fcs_render(cam) (on-demand decode remains
- * the fallback, so the image is the same as without prefetch), then the
+/* One frame with learned prefetch: fcs_render_visible(cam) (on-demand decode
+ * of the bricks the rays enter remains the fallback, so the image is the same
+ * as without prefetch), then the
  * network reads the camera features and the frame's visited-brick mask on the
  * device (n_in = 12 + bricks, n_out = bricks) and every non-empty brick with
  * probability >= threshold that is not resident at its LOD under `cam_ahead`
  * (NULL: `cam`) is decoded ahead with fcs_prefetch_bricks, most probable
- * first, at most max_prefetch. *out_prefetched (nullable) = decodes issued. */
+ * first, at most max_prefetch. *out_prefetched (nullable) = decodes issued;
+ * out_bricks (nullable, max_prefetch entries) lists the bricks issued, one
+ * entry per candidate, 0xFFFFFFFF where the pool had no room. */
 int fcs_render_prefetch(fcs_volume* vol, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead,
                         uint32_t w, uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out,
-                        uint32_t* out_prefetched);
+                        uint32_t* out_prefetched, uint32_t* out_bricks);
 
 #ifdef __cplusplus
 }

@@ -71,8 +71,8 @@ def test_checksum_and_prefetch_frames_identical(gpu_ctx):
     issued = 0
     for i, cam in enumerate(cams):
         ahead = cams[min(i + 1, len(cams) - 1)]
-        _, n = vol.render_prefetch(mlp, cam, W, H, cam_ahead=ahead, threshold=0.0, max_prefetch=nb, want=())
-        issued += n
+        _, pf = vol.render_prefetch(mlp, cam, W, H, cam_ahead=ahead, threshold=0.0, max_prefetch=nb, want=())
+        issued += pf.size
         assert vol.frame_checksum() == sums[i], i
     assert issued > 0
     mlp.close()
