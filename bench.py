@@ -153,9 +153,10 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": None},
         "kernels": kernels,
         "rans_autotune": {"mapping": {1: "lane per brick", 2: "wave per brick (scalar table)",
-                                      3: "wave per brick (ballot)"}.get(tune["rans_mapping"]),
+                                      3: "wave per brick (ballot)", 4: "wave per brick (LDS table)"
+                                      }.get(tune["rans_mapping"]),
                           "lane_ms": tune["tune_lane_ms"], "wave_table_ms": tune["tune_wave_ms"],
-                          "wave_ballot_ms": tune["tune_ballot_ms"]},
+                          "wave_ballot_ms": tune["tune_ballot_ms"], "wave_lds_ms": tune["tune_wave_lds_ms"]},
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -263,7 +264,8 @@ def bench_config3(args):
                      "frac": ach / HBM_PEAK_GBS, "traffic": None},
         "rans_gbs": rans_bytes / (rans_ms * 1e-3) / 1e9,
         "rans_autotune": {"mapping": tune["rans_mapping"], "lane_ms": tune["tune_lane_ms"],
-                          "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"]},
+                          "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],
+                          "wave_lds_ms": tune["tune_wave_lds_ms"]},
         "batched": {"requests": int(nreq), "lods": "uniform 0..%d" % L, "ms": batch_ms,
                     "gvox_s": bvox / (batch_ms * 1e-3) / 1e9, "gbs": bbytes / (batch_ms * 1e-3) / 1e9,
                     "status_ok": bool(np.all(bst == 0))},

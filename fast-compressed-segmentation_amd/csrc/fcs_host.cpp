@@ -431,7 +431,7 @@ int fcs_sync(fcs_ctx* c) {
 int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
   if (!c) return FCS_E_INVALID;
   if (option == FCS_OPT_RANS_MAPPING) {
-    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_BALLOT) return fail(c, FCS_E_INVALID, "bad mapping");
+    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_LDS) return fail(c, FCS_E_INVALID, "bad mapping");
     c->rans_mapping = (int)value;
     return 0;
   }
@@ -656,20 +656,23 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   int mode = c->rans_mapping;
   if (mode == FCS_RANS_MAPPING_AUTO) {
     if (!v->rans_tuned) {
-      // autotuner: both mappings over the whole volume (identical outputs),
+      // autotuner: every mapping over the whole volume (identical outputs),
       // after a one-brick launch of each so code loading is not timed
       RansArgs one = a;
       one.nlist = 1;
-      HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_WAVE, c->stream));
-      HIP_TRY(c, fcs_launch_rans(one, FCS_RANS_LANE, c->stream));
-      float tw = 0, tl = 0, tb = 0;
-      if (int rc = timed(FCS_RANS_WAVE, &tw)) return rc;
-      if (int rc = timed(FCS_RANS_WAVE_BALLOT, &tb)) return rc;
-      if (int rc = timed(FCS_RANS_LANE, &tl)) return rc;
-      c->timing.tune_wave_ms = tw;
-      c->timing.tune_ballot_ms = tb;
-      c->timing.tune_lane_ms = tl;
-      v->rans_tuned = (tw <= tb && tw <= tl) ? FCS_RANS_WAVE : (tb <= tl ? FCS_RANS_WAVE_BALLOT : FCS_RANS_LANE);
+      const int modes[4] = {FCS_RANS_WAVE, FCS_RANS_WAVE_BALLOT, FCS_RANS_WAVE_LDS, FCS_RANS_LANE};
+      for (int md : modes) HIP_TRY(c, fcs_launch_rans(one, md, c->stream));
+      float t[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4; ++k)
+        if (int rc = timed(modes[k], &t[k])) return rc;
+      c->timing.tune_wave_ms = t[0];
+      c->timing.tune_ballot_ms = t[1];
+      c->timing.tune_wave_lds_ms = t[2];
+      c->timing.tune_lane_ms = t[3];
+      int best = 0;
+      for (int k = 1; k < 4; ++k)
+        if (t[k] < t[best]) best = k;
+      v->rans_tuned = modes[best];
     }
     mode = v->rans_tuned;
   }

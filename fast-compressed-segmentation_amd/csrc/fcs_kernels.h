@@ -10,6 +10,7 @@
 #define FCS_RANS_LANE 1  // one lane per brick
 #define FCS_RANS_WAVE 2  // one wave per brick, scalar-cache decode table
 #define FCS_RANS_WAVE_BALLOT 3  // one wave per brick, ballot over a per-lane CDF
+#define FCS_RANS_WAVE_LDS 4  // one wave per brick, decode table in LDS
 
 struct RansArgs {
   const DevBrick* bricks;

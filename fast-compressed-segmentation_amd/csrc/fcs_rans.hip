@@ -14,6 +14,7 @@
 //     vector compare of x against a per-lane CDF + ballot + popcount, and
 //     nibble word k of a 64-word group is kept in lane k, stored 256 B at a time.
 //     Lowest latency per brick: for volumes with few, long bricks.
+//     The decode table is read through the scalar cache, or copied into LDS.
 //   rans_lane_kernel: one lane per brick, 64 bricks per wave, symbol table in
 //     LDS, stream fetched 128 bytes per lane at a time. Highest throughput:
 //     for volumes with many bricks.
@@ -30,8 +31,13 @@ namespace {
 #define RING_HALF 2048u
 #define RING_PEND (RING_HALF / 64u)
 
+// TM: 0 = decode table through the scalar cache, 1 = ballot over a per-lane
+// CDF (no table), 2 = decode table copied into LDS (one LDS round trip per
+// symbol instead of a scalar-cache lookup that can miss to L2)
+template <int TM>
 __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
   __shared__ uint32_t ring[RING_WORDS];
+  __shared__ uint32_t ltab[TM == 2 ? FCS_RANS_M : 1];
   const uint32_t lane = threadIdx.x;
   const uint32_t b = a.list ? a.list[blockIdx.x] : blockIdx.x;
   const DevBrick B = a.bricks[b];
@@ -51,6 +57,12 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
   const int32_t nwords = (int32_t)((E + 3) >> 2);
   int32_t lo = nwords > (int32_t)RING_WORDS ? nwords - (int32_t)RING_WORDS : 0;
   for (int32_t w = lo + (int32_t)lane; w < nwords; w += 64) ring[w & (RING_WORDS - 1)] = words[w];
+  if (TM == 2) {
+    const uint4* t4 = reinterpret_cast<const uint4*>(a.table);
+    uint4* l4 = reinterpret_cast<uint4*>(ltab);
+#pragma unroll
+    for (uint32_t i = 0; i < FCS_RANS_M / 4 / 64; ++i) l4[lane + 64 * i] = t4[lane + 64 * i];
+  }
   uint32_t pend[RING_PEND];
 #pragma unroll
   for (uint32_t i = 0; i < RING_PEND; ++i) {
@@ -114,7 +126,11 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
   auto decode_one = [&]() -> uint32_t {
     const uint32_t x = state & (FCS_RANS_M - 1);
     uint32_t sym;
-    if (tab) {
+    if (TM == 2) {
+      const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)ltab[x]);
+      sym = e >> 25;
+      state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (e & 0xFFFu);
+    } else if (TM == 0) {
       const uint32_t e = tab[x];
       sym = e >> 25;
       state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (e & 0xFFFu);
@@ -303,10 +319,12 @@ __global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
 
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s) {
   if (a.nlist == 0) return hipSuccess;
-  if (mode == FCS_RANS_WAVE || mode == FCS_RANS_WAVE_BALLOT) {
-    RansArgs w = a;
-    if (mode == FCS_RANS_WAVE_BALLOT) w.table = nullptr;
-    hipLaunchKernelGGL(rans_wave_kernel, dim3(a.nlist), dim3(64), 0, s, w);
+  if (mode == FCS_RANS_WAVE) {
+    hipLaunchKernelGGL(rans_wave_kernel<0>, dim3(a.nlist), dim3(64), 0, s, a);
+  } else if (mode == FCS_RANS_WAVE_BALLOT) {
+    hipLaunchKernelGGL(rans_wave_kernel<1>, dim3(a.nlist), dim3(64), 0, s, a);
+  } else if (mode == FCS_RANS_WAVE_LDS) {
+    hipLaunchKernelGGL(rans_wave_kernel<2>, dim3(a.nlist), dim3(64), 0, s, a);
   } else {
     hipLaunchKernelGGL(rans_lane_kernel, dim3((a.nlist + 255) / 256), dim3(256), 0, s, a);
   }

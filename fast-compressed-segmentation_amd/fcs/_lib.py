@@ -102,11 +102,11 @@ class TimingStruct(ctypes.Structure):
         ("tune_lane_ms", ctypes.c_float),
         ("tune_wave_ms", ctypes.c_float),
         ("tune_ballot_ms", ctypes.c_float),
-        ("pad", ctypes.c_float),
+        ("tune_wave_lds_ms", ctypes.c_float),
     ]
 
 
-RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT = 0, 1, 2, 3
+RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT, RANS_WAVE_LDS = 0, 1, 2, 3, 4
 OPT_RANS_MAPPING = 1
 
 

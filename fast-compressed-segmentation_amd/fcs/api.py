@@ -92,7 +92,7 @@ class Context:
         return {f: getattr(t, f) for f, _ in TimingStruct._fields_}
 
     def set_rans_mapping(self, mapping):
-        """0 = autotuned, 1 = one lane per brick, 2 = one wave per brick."""
+        """0 = autotuned, 1 = one lane per brick, 2/3/4 = one wave per brick (scalar-cache table / ballot / LDS table)."""
         check(lib.fcs_set_option(self.h, 1, int(mapping)), self.h)
 
     def sync(self):

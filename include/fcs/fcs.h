@@ -119,15 +119,16 @@ typedef struct {
   float tune_lane_ms;     /* autotuner: whole-volume rANS time, one lane per brick */
   float tune_wave_ms;     /* autotuner: whole-volume rANS time, one wave per brick (table) */
   float tune_ballot_ms;   /* autotuner: whole-volume rANS time, one wave per brick (ballot) */
-  float pad;
+  float tune_wave_lds_ms; /* autotuner: whole-volume rANS time, one wave per brick (LDS table) */
 } fcs_timing;
 
 /* Options (fcs_set_option). */
 #define FCS_OPT_RANS_MAPPING 1 /* FCS_RANS_MAPPING_*; default AUTO */
-#define FCS_RANS_MAPPING_AUTO 0 /* measure both on the volume's first rANS pass, keep the faster */
+#define FCS_RANS_MAPPING_AUTO 0 /* measure every mapping on the volume's first rANS pass, keep the fastest */
 #define FCS_RANS_MAPPING_LANE 1 /* one lane per brick (many bricks) */
 #define FCS_RANS_MAPPING_WAVE 2 /* one wave per brick (few, long bricks), scalar decode table */
 #define FCS_RANS_MAPPING_WAVE_BALLOT 3 /* one wave per brick, ballot over a per-lane CDF */
+#define FCS_RANS_MAPPING_WAVE_LDS 4 /* one wave per brick, decode table in LDS */
 
 int fcs_create(int hip_device, fcs_ctx** out);
 void fcs_destroy(fcs_ctx* ctx);

@@ -131,10 +131,10 @@ def test_config2_full_decode_bit_exact(gpu_ctx, cfg2):
         assert np.array_equal(cache[i, O.pyramid_offset(6):].astype(np.uint64), brick_from_volume(src, i, 64))
 
 
-@pytest.mark.parametrize("mapping", [1, 2, 3])
+@pytest.mark.parametrize("mapping", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["cfg1_b32", "microns8_b64", "checker_b16", "const_mix_b8"])
 def test_both_rans_mappings_bit_exact(gpu_ctx, manifest, name, mapping):
-    """Lane-per-brick (1) and wave-per-brick (2: scalar table, 3: ballot) rANS kernels, forced."""
+    """Lane-per-brick (1) and wave-per-brick (2: scalar table, 3: ballot, 4: LDS table) rANS kernels, forced."""
     print(f"[gpu] rANS mapping {mapping} on {name}", flush=True)
     gpu_ctx.set_rans_mapping(mapping)
     try:
@@ -154,7 +154,7 @@ def test_both_rans_mappings_bit_exact(gpu_ctx, manifest, name, mapping):
 def test_config2_both_mappings(gpu_ctx, cfg2):
     _, data = cfg2
     outs = []
-    for mapping in (1, 2, 3):
+    for mapping in (1, 2, 3, 4):
         gpu_ctx.set_rans_mapping(mapping)
         vol = gpu_ctx.load(data)
         assert np.all(vol.decode_rans() == 0)
@@ -162,4 +162,4 @@ def test_config2_both_mappings(gpu_ctx, cfg2):
         outs.append(vol.read_cache(0, 512 * 262144))
         vol.close()
     gpu_ctx.set_rans_mapping(0)
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
