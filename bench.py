@@ -34,6 +34,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(summary, kernel):
+    """HBM bytes per launch of `kernel` from a committed counter summary
+    (profiles/*.md, lines 'traffic_bytes: <kernel> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes:"):
+                    _, k, b = line.split()
+                    if k == kernel:
+                        return int(b)
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +67,11 @@ def main():
     ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--threshold", type=float, default=0.02)
     ap.add_argument("--every", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU sharding is "
                                    "described in DESIGN.md and not built", "n_gpus": args.gpus}))
@@ -150,7 +170,8 @@ def main():
         "march_ms_per_frame": march_ms,
         "decoded_voxels_per_step": voxels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("hbm_traffic_512cube_b64.md", dom)},
         "kernels": kernels,
         "rans_autotune": {"mapping": {1: "lane per brick", 2: "wave per brick (scalar table)",
                                       3: "wave per brick (ballot)", 4: "wave per brick (LDS table)"
@@ -186,6 +207,27 @@ def main():
             "frames_per_s_1080p_1thread": 1.0 / frame_s,
         }
     print(json.dumps(out))
+
+
+def copy_calibration(nbytes):
+    """Three device-to-device copies of `nbytes` (a plain copy kernel of known
+    size): the FETCH_SIZE / WRITE_SIZE counters of one such dispatch against
+    nbytes give the scale factor for the decoder's counter readings."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
 
 
 def bench_config3(args):
