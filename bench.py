@@ -288,6 +288,7 @@ def bench_config3(args):
         vol.request_bricks(bricks, lods)
         bms.append(ctx.timing()["decode_ms"])
     batch_ms = float(np.median(bms))
+    tops = ctx.timing()
     dec_ms = rans_ms + ops_ms
     ach = ops_bytes / (ops_ms * 1e-3) / 1e9
     out = {
@@ -308,6 +309,8 @@ def bench_config3(args):
         "rans_autotune": {"mapping": tune["rans_mapping"], "lane_ms": tune["tune_lane_ms"],
                           "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],
                           "wave_lds_ms": tune["tune_wave_lds_ms"]},
+        "ops_autotune": {"split": {1: "3 workgroups/CU", 2: "4 workgroups/CU"}.get(tops["ops_split"]),
+                         "3wg_ms": tops["tune_ops_3wg_ms"], "4wg_ms": tops["tune_ops_4wg_ms"]},
         "batched": {"requests": int(nreq), "lods": "uniform 0..%d" % L, "ms": batch_ms,
                     "gvox_s": bvox / (batch_ms * 1e-3) / 1e9, "gbs": bbytes / (batch_ms * 1e-3) / 1e9,
                     "status_ok": bool(np.all(bst == 0))},

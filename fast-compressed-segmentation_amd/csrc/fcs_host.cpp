@@ -60,6 +60,7 @@ struct fcs_volume {
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
   int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
+  int ops_tuned = 0;   // op decoder split picked by the autotuner (FCS_OPS_SPLIT_*, 0 = not yet)
   // brick cache pool (voxels of label_bits/8 bytes)
   void* d_cache = nullptr;
   uint64_t cap = 0, bump = 0;
@@ -191,12 +192,23 @@ int ensure_cache(fcs_volume* v) {
 // statuses are copied back asynchronously (read them after the stream is
 // synchronised, with finish_decode). With `timed`, the launch is bracketed by
 // the context's timing events.
+// The op decoder split used for this volume: the option when forced, else
+// the autotuner's pick (three workgroups per CU until it has run).
+int ops_split_of(const fcs_volume* v) {
+  const int forced = v->ctx->ops_split;
+  if (forced != FCS_OPS_SPLIT_AUTO) return forced;
+  return v->ops_tuned ? v->ops_tuned : FCS_OPS_SPLIT_3WG;
+}
+
 int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, hipStream_t stream,
-                  bool timed) {
+                  bool timed, int split = 0) {
   fcs_ctx* c = v->ctx;
   ln.reqs = reqs;
   const size_t n = reqs.size();
   if (n == 0) return 0;
+  if (split == 0) split = ops_split_of(v);
+  const int ksplit = split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG;
+  c->timing.ops_split = (uint32_t)split;
   if (int rc = ensure_req(v, ln, n)) return rc;
   // Bin the requests by estimated cost, heaviest first (stable within a bin):
   // the persistent decoder deals the front half round-robin and claims the
@@ -225,11 +237,11 @@ int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& r
   if (v->label_bits == 32) {
     OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, ln.d_req,
                         (uint32_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, stream));
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit));
   } else {
     OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, ln.d_req,
                         (uint64_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, stream));
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit));
   }
   if (timed) HIP_TRY(c, hipEventRecord(c->ev[1], stream));
   ln.status_sorted.resize(n);
@@ -246,6 +258,9 @@ void finish_decode(DecodeLane& ln, std::vector<uint32_t>& status) {
   for (size_t i = 0; i < n; ++i) status[ln.perm[i]] = ln.status_sorted[i];
 }
 
+// Smallest batch the op decoder's split autotuner measures on.
+constexpr size_t kOpsTuneMinRequests = 2048;
+
 // Decode `reqs` on the context stream and wait; statuses in caller order.
 int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
                uint64_t* voxels_out) {
@@ -255,11 +270,31 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     c->timing.decode_ms = 0.0f;
     return 0;
   }
-  if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true)) return rc;
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  finish_decode(v->lane_main, status);
   float ms = 0;
-  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  if (c->ops_split == FCS_OPS_SPLIT_AUTO && !v->ops_tuned && v->L <= 5 && reqs.size() >= kOpsTuneMinRequests) {
+    // Autotune the LDS split on this batch: decoding is idempotent (every
+    // request rewrites the same cache slots with the same labels), so the
+    // batch runs three times -- a warm-up with three workgroups per CU, then
+    // four, then three again, timed -- and the faster split is kept.
+    float t[2] = {0, 0};
+    const int order[3] = {FCS_OPS_SPLIT_3WG, FCS_OPS_SPLIT_4WG, FCS_OPS_SPLIT_3WG};
+    for (int k = 0; k < 3; ++k) {
+      if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true, order[k])) return rc;
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+      if (k) t[order[k] - 1] = ms;
+    }
+    c->timing.tune_ops_3wg_ms = t[0];
+    c->timing.tune_ops_4wg_ms = t[1];
+    v->ops_tuned = t[1] < t[0] ? FCS_OPS_SPLIT_4WG : FCS_OPS_SPLIT_3WG;
+    ms = t[v->ops_tuned - 1];
+    c->timing.ops_split = (uint32_t)v->ops_tuned;
+  } else {
+    if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  }
+  finish_decode(v->lane_main, status);
   c->timing.decode_ms = ms;
   uint64_t vox = 0, bytes = 0;
   for (const auto& q : reqs) {
@@ -433,6 +468,11 @@ int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
   if (option == FCS_OPT_RANS_MAPPING) {
     if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_LDS) return fail(c, FCS_E_INVALID, "bad mapping");
     c->rans_mapping = (int)value;
+    return 0;
+  }
+  if (option == FCS_OPT_OPS_SPLIT) {
+    if (value < FCS_OPS_SPLIT_AUTO || value > FCS_OPS_SPLIT_4WG) return fail(c, FCS_E_INVALID, "bad ops split");
+    c->ops_split = (int)value;
     return 0;
   }
   return fail(c, FCS_E_INVALID, "unknown option");

@@ -26,6 +26,7 @@ struct fcs_ctx {
   std::string err;
   fcs_timing timing{};
   int rans_mapping = FCS_RANS_MAPPING_AUTO;
+  int ops_split = FCS_OPS_SPLIT_AUTO;
   std::vector<fcs_volume*> volumes;
   std::vector<fcs_mlp*> mlps;
 };

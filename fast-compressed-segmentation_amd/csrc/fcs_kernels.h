@@ -74,9 +74,13 @@ struct MarchArgs {
 
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s);
 size_t fcs_ops_lds_bytes(uint32_t L);
+// Op decoder LDS splits for bricks up to 32^3 (64^3 bricks have one).
+#define FCS_OPS_3WG 0  // ~52 KB LDS, three workgroups per CU
+#define FCS_OPS_4WG 1  // ~37 KB LDS, four workgroups per CU
 // Persistent launch: min(nreq, resident workgroups per CU x num_cu) workgroups.
 template <typename LT>
-hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr);
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr,
+                          int split = FCS_OPS_3WG);
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s);
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s);

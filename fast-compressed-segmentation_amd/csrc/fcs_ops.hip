@@ -32,8 +32,9 @@
 namespace {
 
 template <uint32_t NT_, uint32_t VCAP_, uint32_t XCAP_, uint32_t BIG_, uint32_t ACT_, bool PACKED_, uint32_t SW_,
-          uint32_t PBYTES_>
+          uint32_t PBYTES_, uint32_t MINW_ = 1>
 struct OpsCfg {
+  static constexpr uint32_t MINW = MINW_;    // waves per SIMD the register budget must allow
   static constexpr uint32_t NT = NT_;        // threads per workgroup
   static constexpr uint32_t NW = NT_ / 64;    // waves
   static constexpr uint32_t VCAP = VCAP_;    // two ping-pong level buffers (u16)
@@ -46,8 +47,12 @@ struct OpsCfg {
   static constexpr uint32_t SW = SW_;        // stop-bit words per level
   static constexpr uint32_t PBYTES = PBYTES_;  // palette bytes staged in LDS
 };
-// bricks up to 32^3: four waves, ~52 KB of LDS (three workgroups per CU for u32 labels)
+// bricks up to 32^3, two LDS splits picked per volume by the runtime's
+// autotuner: ~52 KB (three workgroups per CU, parent values packed into the
+// active list, 12,288 compact ops) or ~37 KB (four workgroups per CU within
+// 128 VGPRs, 8,192 compact ops)
 using CfgSmall = OpsCfg<256, 4096, 8192, 0, 4096, true, 128, 2048>;
+using CfgSmall4 = OpsCfg<256, 4096, 4096, 0, 4096, false, 128, 1024, 4>;
 // 64^3 bricks: sixteen waves, the 32768-node level in LDS (~155 KB)
 using CfgLarge = OpsCfg<1024, 4096, 0, 32768, 32768, false, 1024, 0>;
 
@@ -694,7 +699,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
 // takes the index of request i+3, loads the DecodeReq of request i+2 and the
 // brick record of request i+1; they are stored to LDS at the end of request i.
 template <typename LT, class C>
-__global__ __launch_bounds__(C::NT) void ops_kernel(OpsArgs<LT> a) {
+__global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
   using Y = Layout<C, LT>;
   Lds<LT> S;
@@ -797,10 +802,11 @@ size_t fcs_ops_lds_bytes(uint32_t L) {
 }
 
 template <typename LT>
-hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out) {
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int split) {
   if (a.nreq == 0) return hipSuccess;
   if (a.L > 6) return hipErrorInvalidValue;  // 16-bit node indices: bricks up to 64^3
   if (a.L == 6) return launch_cfg<LT, CfgLarge>(a, num_cu, s, grid_out);
+  if (split == FCS_OPS_4WG) return launch_cfg<LT, CfgSmall4>(a, num_cu, s, grid_out);
   return launch_cfg<LT, CfgSmall>(a, num_cu, s, grid_out);
 }
 
@@ -810,5 +816,5 @@ hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*);
-template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*);
+template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*, int);
+template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*, int);

@@ -103,11 +103,17 @@ class TimingStruct(ctypes.Structure):
         ("tune_wave_ms", ctypes.c_float),
         ("tune_ballot_ms", ctypes.c_float),
         ("tune_wave_lds_ms", ctypes.c_float),
+        ("ops_split", ctypes.c_uint32),
+        ("tune_ops_3wg_ms", ctypes.c_float),
+        ("tune_ops_4wg_ms", ctypes.c_float),
+        ("pad", ctypes.c_uint32),
     ]
 
 
 RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT, RANS_WAVE_LDS = 0, 1, 2, 3, 4
 OPT_RANS_MAPPING = 1
+OPT_OPS_SPLIT = 2
+OPS_SPLIT_AUTO, OPS_SPLIT_3WG, OPS_SPLIT_4WG = 0, 1, 2
 
 
 class GenParams(ctypes.Structure):

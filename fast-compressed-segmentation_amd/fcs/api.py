@@ -95,6 +95,11 @@ class Context:
         """0 = autotuned, 1 = one lane per brick, 2/3/4 = one wave per brick (scalar-cache table / ballot / LDS table)."""
         check(lib.fcs_set_option(self.h, 1, int(mapping)), self.h)
 
+    def set_ops_split(self, split):
+        """Op decoder LDS split for bricks up to 32^3: 0 = autotuned per volume,
+        1 = three workgroups per CU, 2 = four workgroups per CU."""
+        check(lib.fcs_set_option(self.h, 2, int(split)), self.h)
+
     def sync(self):
         check(lib.fcs_sync(self.h), self.h)
 

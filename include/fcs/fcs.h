@@ -120,6 +120,10 @@ typedef struct {
   float tune_wave_ms;     /* autotuner: whole-volume rANS time, one wave per brick (table) */
   float tune_ballot_ms;   /* autotuner: whole-volume rANS time, one wave per brick (ballot) */
   float tune_wave_lds_ms; /* autotuner: whole-volume rANS time, one wave per brick (LDS table) */
+  uint32_t ops_split;     /* FCS_OPS_SPLIT_* the op decoder used last (bricks up to 32^3) */
+  float tune_ops_3wg_ms;  /* autotuner: op decode time of the first large batch, 3 workgroups/CU */
+  float tune_ops_4wg_ms;  /* autotuner: the same batch, 4 workgroups/CU */
+  uint32_t pad;
 } fcs_timing;
 
 /* Options (fcs_set_option). */
@@ -129,6 +133,10 @@ typedef struct {
 #define FCS_RANS_MAPPING_WAVE 2 /* one wave per brick (few, long bricks), scalar decode table */
 #define FCS_RANS_MAPPING_WAVE_BALLOT 3 /* one wave per brick, ballot over a per-lane CDF */
 #define FCS_RANS_MAPPING_WAVE_LDS 4 /* one wave per brick, decode table in LDS */
+#define FCS_OPT_OPS_SPLIT 2 /* FCS_OPS_SPLIT_*; default AUTO */
+#define FCS_OPS_SPLIT_AUTO 0 /* time both splits on a volume's first batch of >= 2048 requests, keep the faster */
+#define FCS_OPS_SPLIT_3WG 1 /* op decoder LDS split for three workgroups per CU (bricks up to 32^3) */
+#define FCS_OPS_SPLIT_4WG 2 /* op decoder LDS split for four workgroups per CU (bricks up to 32^3) */
 
 int fcs_create(int hip_device, fcs_ctx** out);
 void fcs_destroy(fcs_ctx* ctx);

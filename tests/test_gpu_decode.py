@@ -163,3 +163,53 @@ def test_config2_both_mappings(gpu_ctx, cfg2):
         vol.close()
     gpu_ctx.set_rans_mapping(0)
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("name", ["cfg1_b32", "rand_b8", "checker_b16", "u64_b8", "lookback16_b8"])
+def test_ops_lds_splits_bit_exact(gpu_ctx, manifest, name, split):
+    """Both op decoder LDS splits for bricks up to 32^3 (1: three workgroups
+    per CU, 2: four), forced, against the oracle's pyramid."""
+    print(f"[gpu] ops split {split} on {name}", flush=True)
+    gpu_ctx.set_ops_split(split)
+    try:
+        vol, ds = _load(gpu_ctx, name, manifest)
+        assert np.all(vol.decode_rans() == 0)
+        vol.decode_all(pyramid=True)
+        assert gpu_ctx.timing()["ops_split"] == split
+        L = vol.info["levels"]
+        per = O.pyramid_offset(L + 1)
+        cache = vol.read_cache(0, per * vol.info["num_bricks"]).reshape(-1, per).astype(np.uint64)
+        ref, s = ds.decode_all_pyramid()
+        assert s == 0 and np.array_equal(cache, ref)
+        vol.close()
+    finally:
+        gpu_ctx.set_ops_split(0)
+
+
+def test_ops_split_autotune_512cube_b32(gpu_ctx, cfg2):
+    """512^3 re-encoded with 32^3 bricks (4,096 requests): the first large
+    batch times both splits and keeps the faster; the labels equal those of
+    either forced split and the source volume."""
+    src, _ = cfg2
+    data = fcs.encode_volume(src, 32)
+    outs = {}
+    for split in (0, 1, 2):
+        print(f"[gpu] 512^3 b32 full decode, ops split {split}", flush=True)
+        gpu_ctx.set_ops_split(split)
+        vol = gpu_ctx.load(data)
+        assert np.all(vol.decode_rans() == 0)
+        vol.decode_all(lod=5, pyramid=False)
+        t = gpu_ctx.timing()
+        if split == 0:
+            assert t["tune_ops_3wg_ms"] > 0 and t["tune_ops_4wg_ms"] > 0
+            assert t["ops_split"] in (1, 2)
+        else:
+            assert t["ops_split"] == split
+        outs[split] = vol.read_cache(0, 4096 * 32768)
+        vol.close()
+    gpu_ctx.set_ops_split(0)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    cache = outs[0].reshape(4096, 32768)
+    for i in (0, 1000, 4095):
+        assert np.array_equal(cache[i].astype(np.uint64), brick_from_volume(src, i, 32))
