@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -68,6 +69,10 @@ struct fcs_volume {
   std::vector<int16_t> res_lod;   // -1 = not resident
   std::vector<uint8_t> res_pyr;   // 1 = pyramid layout (levels 0..res_lod)
   std::vector<Alloc> res_alloc;
+  uint64_t res_epoch = 1;     // bumped on every residency change
+  uint64_t res_published = 0; // epoch of the device residency table
+  uint64_t* d_res = nullptr;  // per brick residency word (FCS_RES_*), for device-side frame setup
+  uint64_t serial = 0;
   // direct-mapped cache: brick i owns voxels [i * stride, (i + 1) * stride)
   bool direct = false;
   uint64_t stride = 0;
@@ -141,6 +146,7 @@ bool resident_at(const fcs_volume* v, uint32_t i, uint32_t t) {
 void cache_release(fcs_volume* v, uint32_t brick, std::vector<Alloc>* deferred) {
   if (v->res_lod[brick] < 0) return;
   const Alloc a = v->res_alloc[brick];
+  ++v->res_epoch;
   if (v->direct) {
     v->res_lod[brick] = -1;
     return;
@@ -612,6 +618,10 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     }
     if (int rc2 = dev_upload(c, &v->d_rtab, tab.data(), tab.size() * 4)) { fcs_volume_free(v); return rc2; }
   }
+  {
+    static std::atomic<uint64_t> next_serial{1};
+    v->serial = next_serial.fetch_add(1);
+  }
   v->res_lod.assign(v->nb, -1);
   v->res_pyr.assign(v->nb, 0);
   v->res_alloc.assign(v->nb, Alloc{});
@@ -624,6 +634,44 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
 
 }  // extern "C"
 
+fcs_ctx* fcs_volume_context(const fcs_volume* v) { return v ? v->ctx : nullptr; }
+
+int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* o) {
+  if (!v || !v->ctx || !o) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
+  if (!v->d_res) {
+    HIP_TRY(c, hipMalloc((void**)&v->d_res, (size_t)v->nb * 8));
+    v->res_published = 0;
+  }
+  if (v->res_published != v->res_epoch) {
+    std::vector<uint64_t> words(v->nb);
+    for (uint32_t i = 0; i < v->nb; ++i) {
+      uint64_t w = FCS_RES_NONE << FCS_RES_LOD_SHIFT;
+      if (v->res_lod[i] >= 0) {
+        w = (v->res_alloc[i].off & FCS_RES_OFF_MASK) | ((uint64_t)v->res_lod[i] << FCS_RES_LOD_SHIFT);
+        if (v->res_pyr[i]) w |= FCS_RES_PYRAMID;
+      }
+      if (v->empty[i]) w |= FCS_RES_EMPTY;
+      words[i] = w;
+    }
+    HIP_TRY(c, hipDeviceSynchronize());
+    HIP_TRY(c, hipMemcpy(v->d_res, words.data(), words.size() * 8, hipMemcpyHostToDevice));
+    v->res_published = v->res_epoch;
+  }
+  o->serial = v->serial;
+  o->res = v->d_res;
+  o->pal0 = v->d_pal0;
+  o->cache = v->d_cache;
+  o->nb = v->nb;
+  o->b = v->b;
+  o->L = v->L;
+  o->B = v->B;
+  o->label_bits = v->label_bits;
+  return 0;
+}
+
 void fcs_volume_release_device(fcs_volume* v) {
   fcs_ctx* c = v->ctx;
   if (!c) return;
@@ -632,7 +680,7 @@ void fcs_volume_release_device(fcs_volume* v) {
   (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab,
                   v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
-                  v->d_misses, v->d_csum};
+                  v->d_misses, v->d_csum, v->d_res};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   free_lane(v->lane_main);
@@ -759,6 +807,7 @@ int fcs_cache_clear(fcs_volume* v) {
   for (auto& f : v->free_lists) f.clear();
   std::fill(v->res_lod.begin(), v->res_lod.end(), (int16_t)-1);
   std::fill(v->res_pyr.begin(), v->res_pyr.end(), (uint8_t)0);
+  ++v->res_epoch;
   return 0;
 }
 
@@ -822,6 +871,7 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
     v->res_lod[bi] = (int16_t)t;
     v->res_pyr[bi] = 0;
     v->res_alloc[bi] = Alloc{off, sz};
+    ++v->res_epoch;
     req_of[i] = (int64_t)reqs.size();
     reqs.push_back(DecodeReq{bi, t, off});
     if (out_off) out_off[i] = off;
@@ -869,6 +919,7 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
     v->res_lod[i] = (int16_t)t;
     v->res_pyr[i] = pyramid ? 1 : 0;
     v->res_alloc[i] = Alloc{off, per};
+    ++v->res_epoch;
   }
   v->bump = per * v->nb;
   std::vector<uint32_t> rs;
@@ -1007,12 +1058,12 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
   memcpy(&mc, cam, sizeof mc);
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
   if (v->label_bits == 32) {
-    MarchArgs<uint32_t> a{mc, v->d_meta, v->d_offs, (const uint32_t*)v->d_pal0, (const uint32_t*)v->d_cache,
+    MarchArgs<uint32_t> a{mc, nullptr, v->d_meta, v->d_offs, (const uint32_t*)v->d_pal0, (const uint32_t*)v->d_cache,
                           (uint32_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
     HIP_TRY(c, fcs_launch_march<uint32_t>(a, c->stream));
   } else {
-    MarchArgs<uint64_t> a{mc, v->d_meta, v->d_offs, (const uint64_t*)v->d_pal0, (const uint64_t*)v->d_cache,
+    MarchArgs<uint64_t> a{mc, nullptr, v->d_meta, v->d_offs, (const uint64_t*)v->d_pal0, (const uint64_t*)v->d_cache,
                           (uint64_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
     HIP_TRY(c, fcs_launch_march<uint64_t>(a, c->stream));
@@ -1144,6 +1195,7 @@ int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod
     v->res_lod[bi] = (int16_t)t;
     v->res_pyr[bi] = 0;
     v->res_alloc[bi] = Alloc{off, sz};
+    ++v->res_epoch;
     reqs.push_back(DecodeReq{bi, t, off});
   }
   if (reqs.empty()) {

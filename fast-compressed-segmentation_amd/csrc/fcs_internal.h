@@ -55,3 +55,19 @@ struct fcs_mlp;
 int fcs_mlp_predict_dev(fcs_mlp* m, const float* pose, uint32_t n_pose, const uint32_t* d_mask, const float** d_probs);
 uint32_t fcs_mlp_outputs(const fcs_mlp* m);
 uint32_t fcs_mlp_inputs(const fcs_mlp* m);
+
+// Device-side description of a volume for frames launched outside the
+// volume's own calls (views). `serial` identifies the volume for the lifetime
+// of the process (addresses can be reused).
+struct fcs_volume_desc {
+  uint64_t serial;
+  const uint64_t* res;  // per brick residency word (FCS_RES_* in fcs_kernels.h)
+  const void* pal0;     // per brick palette[0]
+  const void* cache;    // brick cache pool (may be NULL: nothing resident)
+  uint32_t nb, b, L, B, label_bits;
+};
+// Bring the volume's device residency table up to date (after settling its
+// prefetches; when it changed, the device is synchronised first because
+// frames in flight may read the old table) and describe the volume.
+int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* out);
+fcs_ctx* fcs_volume_context(const fcs_volume* v);

@@ -58,6 +58,7 @@ struct MarchCam {
 template <typename LT>
 struct MarchArgs {
   MarchCam cam;
+  const MarchCam* cam_dev;  // nullable: when set, the camera is read from device memory instead of `cam`
   const uint32_t* meta;   // per brick: lod | EMPTY | RESIDENT
   const uint64_t* offs;   // per brick: cache voxel offset of its labels at `lod`
   const LT* pal0;         // per brick: palette[0]
@@ -72,6 +73,27 @@ struct MarchArgs {
   uint32_t brick_size, bricks_per_axis;
 };
 
+// Device residency word of a brick: cache voxel offset (bits 0-47), resident
+// LOD (bits 48-55, 0xFF = not resident), pyramid layout (bit 56), empty (bit 57).
+#define FCS_RES_OFF_MASK 0xFFFFFFFFFFFFull
+#define FCS_RES_LOD_SHIFT 48
+#define FCS_RES_NONE 0xFFull
+#define FCS_RES_PYRAMID (1ull << 56)
+#define FCS_RES_EMPTY (1ull << 57)
+
+// Per-frame brick metadata on the device: the LOD rule of camera.js:103-137
+// (float64, same operation order as the host rule) or a fixed LOD per brick,
+// then meta/offs as the marcher reads them.
+struct FrameSetupArgs {
+  const uint64_t* res;     // per brick residency word
+  const MarchCam* cam;     // device camera
+  const uint8_t* lod_fixed;  // nullable: per-brick LOD instead of the rule
+  uint32_t* meta;
+  uint64_t* offs;
+  uint32_t nb, b, L, w, h;
+};
+
+hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s);
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s);
 size_t fcs_ops_lds_bytes(uint32_t L);
 // Op decoder LDS splits for bricks up to 32^3 (64^3 bricks have one).

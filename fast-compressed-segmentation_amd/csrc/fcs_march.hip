@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= a.w || py >= a.h) return;
+  if (a.cam_dev) a.cam = *a.cam_dev;
   const size_t pix = (size_t)py * a.w + px;
   LT hit_label = 0;
   uint64_t hit_vox = ~0ull;
@@ -220,6 +221,58 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
 }
 
 }  // namespace
+
+namespace {
+// One thread per brick. The LOD rule is calculateBrickLOD (camera.js:103-137)
+// in float64 with the host rule's exact operation order: focal =
+// (min(w,h)/2)/tan(fov/2), d = |brick centre - camera| clamped to 0.1, the
+// largest t with (focal * b/2^t)/d >= 1. Division and square root are the
+// correctly rounded IEEE operations on both sides and nothing is contracted,
+// so device and host pick the same LOD for every brick.
+__global__ __launch_bounds__(256) void frame_setup_kernel(FrameSetupArgs a) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.nb) return;
+  uint32_t lod;
+  if (a.lod_fixed) {
+    lod = a.lod_fixed[i] < a.L ? a.lod_fixed[i] : a.L;
+  } else {
+    const MarchCam& c = *a.cam;
+    const double sw = (double)(a.w < a.h ? a.w : a.h);
+    const double focal = (sw / 2.0) / (double)c.tan_half_fov;
+    const double half = (double)a.b / 2.0;
+    const double cx = (double)fcs_unmorton_x(i) * (double)a.b + half;
+    const double cy = (double)fcs_unmorton_y(i) * (double)a.b + half;
+    const double cz = (double)fcs_unmorton_z(i) * (double)a.b + half;
+    const double dx = cx - (double)c.position[0], dy = cy - (double)c.position[1], dz = cz - (double)c.position[2];
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d < 0.1) d = 0.1;
+    lod = 0;
+    for (uint32_t l = 0; l <= a.L; ++l) {
+      const double voxel = (double)a.b / (double)(1u << l);  // exact power of two
+      if ((focal * voxel) / d >= 1.0) lod = l; else break;
+    }
+  }
+  const uint64_t r = a.res[i];
+  const uint32_t rl = (uint32_t)((r >> FCS_RES_LOD_SHIFT) & 0xFF);
+  const bool pyr = (r & FCS_RES_PYRAMID) != 0;
+  const bool resident = rl != FCS_RES_NONE && (pyr ? rl >= lod : rl == lod);
+  uint32_t m = lod;
+  if (r & FCS_RES_EMPTY) m |= FCS_META_EMPTY;
+  uint64_t off = 0;
+  if (resident) {
+    m |= FCS_META_RESIDENT;
+    off = (r & FCS_RES_OFF_MASK) + (pyr ? fcs_pyramid_level_offset(lod) : 0);
+  }
+  a.meta[i] = m;
+  a.offs[i] = off;
+}
+}  // namespace
+
+hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s) {
+  if (!a.nb) return hipSuccess;
+  hipLaunchKernelGGL(frame_setup_kernel, dim3((a.nb + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
 
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {

@@ -30,4 +30,4 @@ from .api import (  # noqa: F401
 )
 from .producer import Generator, encode_volume, encode_generated, brick_pyramid  # noqa: F401
 from .mlp import MLP, pack_bits  # noqa: F401
-from . import paths, prefetch  # noqa: F401
+from . import paths, prefetch, serve  # noqa: F401

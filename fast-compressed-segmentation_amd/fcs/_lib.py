@@ -186,6 +186,60 @@ _sig("fcs_free_buffer", None, c_vp)
 _sig("fcs_brick_pyramid", ctypes.c_int, c_u64p, ctypes.c_uint32, c_u64p, c_u8p)
 
 
+class ViewResultStruct(ctypes.Structure):
+    _fields_ = [
+        ("checksum", ctypes.c_uint64),
+        ("misses", ctypes.c_uint32),
+        ("w", ctypes.c_uint32),
+        ("h", ctypes.c_uint32),
+        ("label_bits", ctypes.c_uint32),
+        ("device_ms", ctypes.c_float),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+class ServeRequestStruct(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("session", ctypes.c_uint32),
+        ("volume", ctypes.c_uint32),
+        ("w", ctypes.c_uint32),
+        ("h", ctypes.c_uint32),
+        ("cam", CameraStruct),
+    ]
+
+
+class ServeResultStruct(ctypes.Structure):
+    _fields_ = [
+        ("issue_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("checksum", ctypes.c_uint64),
+        ("device_ms", ctypes.c_float),
+        ("misses", ctypes.c_uint32),
+        ("view", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+VIEW_GRAPH, VIEW_LABELS, VIEW_CHECKSUM = 1, 2, 4
+
+_sig("fcs_view_create", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(c_vp))
+_sig("fcs_view_destroy", None, c_vp)
+_sig("fcs_view_render_async", ctypes.c_int, c_vp, c_vp, ctypes.POINTER(CameraStruct), c_u8p, ctypes.c_uint32,
+     ctypes.c_uint32)
+_sig("fcs_view_query", ctypes.c_int, c_vp)
+_sig("fcs_view_wait", ctypes.c_int, c_vp, ctypes.POINTER(ViewResultStruct))
+_sig("fcs_view_labels", c_vp, c_vp)
+_sig("fcs_view_read", ctypes.c_int, c_vp, c_vp, c_u64p, ctypes.POINTER(ctypes.c_float), c_u8p)
+_sig("fcs_view_graphs", ctypes.c_uint32, c_vp)
+_sig("fcs_serve_plan", ctypes.c_int, c_u64p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint64,
+     c_u32p, c_u64p)
+_sig("fcs_serve_admit", ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+     ctypes.c_uint64, c_u32p, c_u64p)
+_sig("fcs_serve_run", ctypes.c_int, ctypes.POINTER(c_vp), ctypes.c_uint32, ctypes.POINTER(c_vp), ctypes.c_uint32,
+     ctypes.POINTER(ServeRequestStruct), ctypes.c_uint32, ctypes.POINTER(ServeResultStruct))
+
+
 def check(rc, ctx=None):
     if rc < 0:
         msg = lib.fcs_last_error(ctx).decode() if ctx else ""

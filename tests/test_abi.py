@@ -7,7 +7,8 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", "fcs", h) for h in ("fcs.h", "fcs_producer.h", "fcs_mlp.h")]
+HEADERS = [os.path.join(ROOT, "include", "fcs", h) for h in ("fcs.h", "fcs_producer.h", "fcs_mlp.h",
+                                                                  "fcs_serve.h")]
 
 
 def declared_functions():
