@@ -11,6 +11,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <cstdio>
 #include <thread>
 #include <unordered_map>
 #include <unordered_set>
@@ -591,9 +592,14 @@ int encode_common(uint32_t BX, uint32_t BY, uint32_t BZ, uint32_t b, const fcs_e
   const Model M = build_model(raw);
 
   // Pass 2: every brick, then rANS.
+  std::atomic<size_t> finished{0};
   parallel_for(nb, threads, [&](size_t i) {
     if (i % 512 != 0) make_brick(i, bricks[i]);
     rans_encode(M, bricks[i]);
+    if (o->progress) {
+      const size_t k = finished.fetch_add(1) + 1;
+      if (k * 16 / nb != (k - 1) * 16 / nb) fprintf(stderr, "[encode] %zu of %zu bricks\n", k, nb);
+    }
   });
 
   size_t total = 16 + 8 + 64 + 64;

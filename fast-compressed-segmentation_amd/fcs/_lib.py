@@ -131,7 +131,8 @@ class GenParams(ctypes.Structure):
 
 
 class EncodeOpts(ctypes.Structure):
-    _fields_ = [("brick_size", ctypes.c_uint32), ("header_brick_size_64", ctypes.c_uint32), ("threads", ctypes.c_int)]
+    _fields_ = [("brick_size", ctypes.c_uint32), ("header_brick_size_64", ctypes.c_uint32), ("threads", ctypes.c_int),
+                ("progress", ctypes.c_uint32)]
 
 
 def _sig(name, res, *args):

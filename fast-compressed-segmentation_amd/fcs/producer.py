@@ -42,7 +42,17 @@ class Generator:
 
 
 def _threads(t):
-    return t if t > 0 else (os.cpu_count() or 4)
+    """Explicit count, else OMP_NUM_THREADS when set (the share of a shared
+    machine), else the CPUs this process may run on."""
+    if t > 0:
+        return t
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 4
 
 
 def encode_volume(labels, brick_size, header_brick_size_64=False, threads=0):
@@ -62,9 +72,10 @@ def encode_volume(labels, brick_size, header_brick_size_64=False, threads=0):
         lib.fcs_free_buffer(buf)
 
 
-def encode_generated(gen, brick_size, header_brick_size_64=False, threads=0):
-    """Encode a generated cube brick by brick (the volume is never materialised)."""
-    o = EncodeOpts(int(brick_size), 1 if header_brick_size_64 else 0, _threads(threads))
+def encode_generated(gen, brick_size, header_brick_size_64=False, threads=0, progress=False):
+    """Encode a generated cube brick by brick (the volume is never materialised).
+    progress: the encoder reports every sixteenth of the bricks on stderr."""
+    o = EncodeOpts(int(brick_size), 1 if header_brick_size_64 else 0, _threads(threads), 1 if progress else 0)
     buf = c_u8p()
     n = ctypes.c_size_t()
     check(lib.fcs_encode_generated(gen.h, ctypes.byref(o), ctypes.byref(buf), ctypes.byref(n)))

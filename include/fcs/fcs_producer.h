@@ -68,6 +68,7 @@ typedef struct {
   uint32_t brick_size;          /* b = 2^L, 2 <= b <= 64 */
   uint32_t header_brick_size_64;/* 1: write 64 in the header like the reference encoder */
   int threads;                  /* <= 0: hardware concurrency */
+  uint32_t progress;            /* 1: print a line to stderr at every sixteenth of the bricks encoded */
 } fcs_encode_opts;
 
 /* Encode a flat label volume (x-fastest, data[x + W*(y + H*z)], W/H/D need
