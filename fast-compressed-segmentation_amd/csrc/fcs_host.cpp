@@ -73,6 +73,19 @@ struct fcs_volume {
   uint64_t res_published = 0; // epoch of the device residency table
   uint64_t* d_res = nullptr;  // per brick residency word (FCS_RES_*), for device-side frame setup
   uint64_t serial = 0;
+  // shaded frames: every palette entry (host copy), the transfer function
+  // (ascending labels, default and current colours, device hash table) and
+  // the lighting
+  std::vector<uint64_t> pal_host;
+  bool tf_built = false, tf_dirty = true;
+  std::vector<uint64_t> tf_labels;
+  std::vector<uint32_t> tf_base, tf_cur;
+  uint64_t* d_tf_keys = nullptr;
+  uint32_t* d_tf_rgba = nullptr;
+  uint32_t tf_mask = 0;
+  float light[3] = {0.0f, 0.0f, 0.0f};
+  float ambient = 0.5f, diffuse = 0.8f;
+  uint32_t* d_frgba = nullptr;  // frame RGBA (frame_cap pixels)
   // direct-mapped cache: brick i owns voxels [i * stride, (i + 1) * stride)
   bool direct = false;
   uint64_t stride = 0;
@@ -334,6 +347,72 @@ int settle_prefetch(fcs_volume* v) {
   return 0;
 }
 
+// Lighting uniform of webgpuSetup.js:205-219: the angle in degrees becomes
+// the direction (sin, 0.6, -cos), computed in double like the JavaScript and
+// rounded to float like its Float32Array upload.
+void set_lighting(fcs_volume* v, float angle_deg, float ambient, float diffuse) {
+  const double a = ((double)angle_deg * 3.141592653589793) / 180.0;
+  v->light[0] = (float)std::sin(a);
+  v->light[1] = 0.6f;
+  v->light[2] = (float)-std::cos(a);
+  v->ambient = ambient;
+  v->diffuse = diffuse;
+}
+
+// Math.round for a non-negative double (halves go up)
+double js_round(double x) {
+  const double f = std::floor(x);
+  return (x - f >= 0.5) ? f + 1.0 : f;
+}
+
+// Build the transfer function on first use (buildTransferFunctionBuffers,
+// webgpuSetup.js:389-458) and upload its hash table when colours changed.
+int ensure_tf(fcs_volume* v) {
+  fcs_ctx* c = v->ctx;
+  if (!v->tf_built) {
+    std::vector<uint64_t> labs;
+    labs.reserve(v->pal_host.size());
+    for (uint64_t x : v->pal_host)
+      if (x != 0) labs.push_back(x);
+    std::sort(labs.begin(), labs.end());
+    labs.erase(std::unique(labs.begin(), labs.end()), labs.end());
+    v->tf_labels = std::move(labs);
+    v->tf_base.resize(v->tf_labels.size());
+    for (size_t i = 0; i < v->tf_labels.size(); ++i) v->tf_base[i] = fcs_tf_default_rgba(i);
+    v->tf_cur = v->tf_base;
+    v->tf_built = true;
+    v->tf_dirty = true;
+  }
+  if (!v->tf_dirty) return 0;
+  const size_t n = v->tf_labels.size();
+  uint32_t cap = 16;
+  while (cap < 2 * n) cap <<= 1;
+  std::vector<uint64_t> keys(cap, 0);
+  std::vector<uint32_t> cols(cap, 0);
+  const uint32_t mask = cap - 1;
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t h = fcs_tf_slot(v->tf_labels[i], mask);
+    while (keys[h] != 0) h = (h + 1) & mask;
+    keys[h] = v->tf_labels[i];
+    cols[h] = v->tf_cur[i];
+  }
+  if (v->tf_mask != mask || !v->d_tf_keys) {
+    if (v->d_tf_keys) (void)hipFree(v->d_tf_keys);
+    if (v->d_tf_rgba) (void)hipFree(v->d_tf_rgba);
+    v->d_tf_keys = nullptr;
+    v->d_tf_rgba = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&v->d_tf_keys, (size_t)cap * 8));
+    HIP_TRY(c, hipMalloc((void**)&v->d_tf_rgba, (size_t)cap * 4));
+    v->tf_mask = mask;
+  }
+  // frames that read the table run on the context stream, synchronously
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(v->d_tf_keys, keys.data(), (size_t)cap * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(v->d_tf_rgba, cols.data(), (size_t)cap * 4, hipMemcpyHostToDevice));
+  v->tf_dirty = false;
+  return 0;
+}
+
 // calculateBrickLOD (camera.js:103-137) for one brick: the largest LOD t
 // with focal * (b / 2^t) / d >= 1, d = distance from the camera to the brick
 // centre, focal from the camera's tan(fov/2) and screen width min(w, h)
@@ -586,6 +665,7 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     v->pal0[i] = hp[v->hb[i].pal_off];
   }
   v->label_bits = wide ? 64 : 32;
+  set_lighting(v, 25.0f, 0.5f, 0.8f);
   int rc = 0;
   if ((rc = dev_upload(c, &v->d_stream, hs.data(), hs.size())) ||
       (rc = dev_upload(c, &v->d_bricks, v->hb.data(), v->hb.size() * sizeof(DevBrick)))) {
@@ -618,6 +698,7 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     }
     if (int rc2 = dev_upload(c, &v->d_rtab, tab.data(), tab.size() * 4)) { fcs_volume_free(v); return rc2; }
   }
+  v->pal_host = std::move(hp);
   {
     static std::atomic<uint64_t> next_serial{1};
     v->serial = next_serial.fetch_add(1);
@@ -680,7 +761,7 @@ void fcs_volume_release_device(fcs_volume* v) {
   (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab,
                   v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
-                  v->d_misses, v->d_csum, v->d_res};
+                  v->d_misses, v->d_csum, v->d_res, v->d_tf_keys, v->d_tf_rgba, v->d_frgba};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   free_lane(v->lane_main);
@@ -971,6 +1052,70 @@ int fcs_brick_lods(const fcs_volume* v, const fcs_camera* cam, uint32_t w, uint3
   return 0;
 }
 
+uint32_t fcs_tf_default_rgba(uint64_t index) {
+  const double i = (double)index;
+  const double hue = std::fmod(i * 137.5, 360.0);
+  const double sat = 0.55 + (double)(index % 3) * 0.08;
+  const double lig = 0.35 + (double)(index % 4) * 0.05;
+  const double opa = 0.25 + (double)(index % 5) * 0.08;
+  // hslToRgb (webgpuSetup.js:490-515)
+  const double cc = (1.0 - std::fabs(2.0 * lig - 1.0)) * sat;
+  const double x = cc * (1.0 - std::fabs(std::fmod(hue / 60.0, 2.0) - 1.0));
+  const double m = lig - cc / 2.0;
+  double r, g, b;
+  if (hue < 60) { r = cc; g = x; b = 0; }
+  else if (hue < 120) { r = x; g = cc; b = 0; }
+  else if (hue < 180) { r = 0; g = cc; b = x; }
+  else if (hue < 240) { r = 0; g = x; b = cc; }
+  else if (hue < 300) { r = x; g = 0; b = cc; }
+  else { r = cc; g = 0; b = x; }
+  const double R = js_round((r + m) * 255.0), Gc = js_round((g + m) * 255.0), Bc = js_round((b + m) * 255.0);
+  const uint32_t r8 = (uint32_t)std::min(std::floor(R * 0.92), 255.0);
+  const uint32_t g8 = (uint32_t)std::min(std::floor(Gc * 0.92), 255.0);
+  const uint32_t b8 = (uint32_t)std::min(std::floor(Bc * 0.92), 255.0);
+  const uint32_t a8 = (uint32_t)std::floor(opa * 255.0);
+  return r8 | (g8 << 8) | (b8 << 16) | (a8 << 24);
+}
+
+int fcs_set_lighting(fcs_volume* v, const fcs_lighting* l) {
+  if (!v || !v->ctx || !l) return FCS_E_INVALID;
+  set_lighting(v, l->light_angle_deg, l->ambient, l->diffuse_strength);
+  return 0;
+}
+
+int fcs_set_label_visibility(fcs_volume* v, const uint64_t* hidden, uint32_t n_hidden, const uint64_t* highlight) {
+  if (!v || !v->ctx || (n_hidden && !hidden)) return FCS_E_INVALID;
+  if (int rc = ensure_tf(v)) return rc;
+  std::vector<uint64_t> hs(hidden, hidden + n_hidden);
+  std::sort(hs.begin(), hs.end());
+  v->tf_cur = v->tf_base;
+  for (size_t i = 0; i < v->tf_labels.size(); ++i) {
+    const uint64_t lab = v->tf_labels[i];
+    if (std::binary_search(hs.begin(), hs.end(), lab)) {
+      v->tf_cur[i] = 0u;
+    } else if (highlight && lab == *highlight) {
+      // rebuildLabelColors (webgpuSetup.js:840-848)
+      const uint32_t c0 = v->tf_base[i];
+      const uint32_t r = (uint32_t)std::floor(std::min(255.0, (double)(c0 & 0xFFu) * 1.8));
+      const uint32_t g = (uint32_t)std::floor(std::min(255.0, (double)((c0 >> 8) & 0xFFu) * 1.8));
+      const uint32_t b = (uint32_t)std::floor(std::min(255.0, (double)((c0 >> 16) & 0xFFu) * 1.8));
+      v->tf_cur[i] = r | (g << 8) | (b << 16) | (c0 & 0xFF000000u);
+    }
+  }
+  v->tf_dirty = true;
+  return ensure_tf(v);
+}
+
+int64_t fcs_transfer_function(fcs_volume* v, uint64_t* labels, uint32_t* rgba, uint64_t cap) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  if (int rc = ensure_tf(v)) return rc;
+  const uint64_t n = v->tf_labels.size();
+  const uint64_t k = std::min<uint64_t>(n, cap);
+  if (labels) std::copy(v->tf_labels.begin(), v->tf_labels.begin() + k, labels);
+  if (rgba) std::copy(v->tf_cur.begin(), v->tf_cur.begin() + k, rgba);
+  return (int64_t)n;
+}
+
 int fcs_camera_orbit(const fcs_volume* v, double yaw, double pitch, double radius, double fov, fcs_camera* o) {
   if (!v || !v->ctx || !o) return FCS_E_INVALID;
   // OrbitController.apply (orbitController.js:100-123) around the volume
@@ -1016,10 +1161,26 @@ int frame_lods(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
   return fcs_brick_lods(v, cam, w, h, lods.data());
 }
 
+// Transfer function, lighting and RGBA output of a shaded march.
+template <typename LT>
+void shade_args(const fcs_volume* v, bool rgba, MarchArgs<LT>& a) {
+  if (!rgba) return;
+  a.out_rgba = v->d_frgba;
+  a.tf_keys = v->d_tf_keys;
+  a.tf_rgba = v->d_tf_rgba;
+  a.tf_mask = v->tf_mask;
+  for (int k = 0; k < 3; ++k) a.light[k] = v->light[k];
+  a.ambient = v->ambient;
+  a.diffuse = v->diffuse;
+}
+
 // Upload the frame's per-brick metadata (LOD, empty, resident at that LOD,
 // cache offset) and march every pixel on the context stream (not waited for).
-int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h) {
+int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h,
+               bool rgba) {
   fcs_ctx* c = v->ctx;
+  if (rgba)
+    if (int rc = ensure_tf(v)) return rc;
   std::vector<uint32_t> meta(v->nb);
   std::vector<uint64_t> offs(v->nb, 0);
   for (uint32_t i = 0; i < v->nb; ++i) {
@@ -1033,18 +1194,20 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
   }
   const size_t npx = (size_t)w * h;
   if (npx > v->frame_cap) {
-    void* ptrs[] = {v->d_flabel, v->d_fvoxel, v->d_ft};
+    void* ptrs[] = {v->d_flabel, v->d_fvoxel, v->d_ft, v->d_frgba};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
     v->d_flabel = nullptr;
     v->d_fvoxel = nullptr;
     v->d_ft = nullptr;
+    v->d_frgba = nullptr;
     v->frame_cap = 0;
     HIP_TRY(c, hipMalloc(&v->d_flabel, npx * v->lbytes()));
     HIP_TRY(c, hipMalloc((void**)&v->d_fvoxel, npx * 8));
     HIP_TRY(c, hipMalloc((void**)&v->d_ft, npx * 4));
     v->frame_cap = npx;
   }
+  if (rgba && !v->d_frgba) HIP_TRY(c, hipMalloc((void**)&v->d_frgba, v->frame_cap * 4));
   if (!v->d_visited) {
     HIP_TRY(c, hipMalloc((void**)&v->d_visited, (size_t)v->nb * 4));
     HIP_TRY(c, hipMalloc((void**)&v->d_misses, 64));
@@ -1061,11 +1224,13 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
     MarchArgs<uint32_t> a{mc, nullptr, v->d_meta, v->d_offs, (const uint32_t*)v->d_pal0, (const uint32_t*)v->d_cache,
                           (uint32_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
+    shade_args(v, rgba, a);
     HIP_TRY(c, fcs_launch_march<uint32_t>(a, c->stream));
   } else {
     MarchArgs<uint64_t> a{mc, nullptr, v->d_meta, v->d_offs, (const uint64_t*)v->d_pal0, (const uint64_t*)v->d_cache,
                           (uint64_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
+    shade_args(v, rgba, a);
     HIP_TRY(c, fcs_launch_march<uint64_t>(a, c->stream));
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
@@ -1079,6 +1244,7 @@ int finish_frame(fcs_volume* v, const std::vector<uint8_t>& lods, uint32_t w, ui
   if (out->label) HIP_TRY(c, hipMemcpyAsync(out->label, v->d_flabel, npx * v->lbytes(), hipMemcpyDeviceToHost, c->stream));
   if (out->voxel) HIP_TRY(c, hipMemcpyAsync(out->voxel, v->d_fvoxel, npx * 8, hipMemcpyDeviceToHost, c->stream));
   if (out->t_hit) HIP_TRY(c, hipMemcpyAsync(out->t_hit, v->d_ft, npx * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out->rgba) HIP_TRY(c, hipMemcpyAsync(out->rgba, v->d_frgba, npx * 4, hipMemcpyDeviceToHost, c->stream));
   if (out->brick_visited)
     HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
   if (out->brick_lod_used) memcpy(out->brick_lod_used, lods.data(), v->nb);
@@ -1128,7 +1294,7 @@ int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
   out->decode_ms = 0.0f;
   out->march_ms = 0.0f;
   if (int rc = decode_list(v, qb, ql, out)) return rc;
-  if (int rc = march_pass(v, cam, lods, w, h)) return rc;
+  if (int rc = march_pass(v, cam, lods, w, h, out->rgba != nullptr)) return rc;
   return finish_frame(v, lods, w, h, out);
 }
 
@@ -1149,7 +1315,7 @@ int fcs_render_visible(fcs_volume* v, const fcs_camera* cam, const uint8_t* bric
   // were missing and march again until no ray misses. The last pass sees
   // every brick its rays enter, so the frame equals fcs_render's.
   for (int pass = 0;; ++pass) {
-    if (int rc = march_pass(v, cam, lods, w, h)) return rc;
+    if (int rc = march_pass(v, cam, lods, w, h, out->rgba != nullptr)) return rc;
     uint32_t misses = 0;
     HIP_TRY(c, hipMemcpyAsync(&misses, v->d_misses, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -71,7 +71,23 @@ struct MarchArgs {
   uint32_t w, h;
   float wf, hf, aspect, grid;
   uint32_t brick_size, bricks_per_axis;
+  // composited, shaded output (the RGBA form of the march; zero = first hit only)
+  uint32_t* out_rgba;
+  const uint64_t* tf_keys;  // transfer function hash table: label (0 = empty slot) ...
+  const uint32_t* tf_rgba;  // ... -> RGBA8
+  uint32_t tf_mask;         // slots - 1 (a power of two minus one)
+  float light[3];           // light direction (towards the light, not normalised)
+  float ambient, diffuse;
 };
+
+// Transfer function hash slot of a label (SplitMix64 finaliser).
+__host__ __device__ __forceinline__ uint32_t fcs_tf_slot(uint64_t label, uint32_t mask) {
+  uint64_t z = label;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (uint32_t)z & mask;
+}
 
 // Device residency word of a brick: cache voxel offset (bits 0-47), resident
 // LOD (bits 48-55, 0xFF = not resident), pyramid layout (bit 56), empty (bit 57).
@@ -103,6 +119,8 @@ size_t fcs_ops_lds_bytes(uint32_t L);
 template <typename LT>
 hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr,
                           int split = FCS_OPS_3WG);
+// First-hit march; with a.out_rgba set, the composited shaded march that also
+// writes the first-hit outputs.
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s);
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s);

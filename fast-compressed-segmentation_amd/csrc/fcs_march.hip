@@ -33,7 +33,144 @@ __device__ __forceinline__ float advance3(int& v0, int& v1, int& v2, float& m0, 
   return t;
 }
 
+// ------------------------------------------------------------ shading
+// The RGBA form follows rayTrace.wgsl:565-906 past the first hit: labels are
+// coloured by the transfer function (decode.wgsl:135-172 lookup, hash colour
+// for labels outside it), samples with alpha > 0.05 are composited front to
+// back with the LOD opacity correction 1 - (1 - a)^stride, samples with
+// alpha > 0.35 are shaded from an LOD-stride central-difference gradient
+// (computeGradientLOD, 428-455) with a diffuse term and, above alpha 0.4, a
+// shadow ray (traceShadowRay, 329-395); the march ends at alpha > 0.95.
+// Samples of bricks that are not resident at their frame LOD read as
+// transparent. (1 - a)^stride is computed by repeated squaring (exact same
+// order on host and device); the shadow falloff uses expf.
+
+__device__ __forceinline__ uint32_t hash_colour(uint64_t label) {
+  // decode.wgsl:149-171
+  uint32_t lo = (uint32_t)label, hi = (uint32_t)(label >> 32);
+  lo = lo ^ hi;
+  lo = lo ^ (lo >> 16);
+  lo = lo * 0x7feb352du;
+  lo = lo ^ (lo >> 15);
+  hi = hi ^ lo;
+  hi = hi ^ (hi >> 16);
+  hi = hi * 0x85ebca6bu;
+  hi = hi ^ (hi >> 13);
+  return (lo & 0xFFu) | (((lo >> 8) & 0xFFu) << 8) | (((hi >> 8) & 0xFFu) << 16) | (0x40u << 24);
+}
+
 template <typename LT>
+__device__ __forceinline__ uint32_t tf_colour(const MarchArgs<LT>& a, LT label) {
+  if (label == 0) return 0u;
+  const uint64_t key = (uint64_t)label;
+  uint32_t h = fcs_tf_slot(key, a.tf_mask);
+  for (uint32_t probe = 0; probe <= a.tf_mask; ++probe) {
+    const uint64_t k = a.tf_keys[h];
+    if (k == key) return a.tf_rgba[h];
+    if (k == 0) break;
+    h = (h + 1) & a.tf_mask;
+  }
+  return hash_colour(key);
+}
+
+__device__ __forceinline__ float alpha_of(uint32_t rgba) { return (float)((rgba >> 24) & 0xFFu) / 255.0f; }
+
+// x^n for an integer n >= 1 by repeated squaring
+__device__ __forceinline__ float powi(float x, uint32_t n) {
+  float r = 1.0f, p = x;
+  while (n) {
+    if (n & 1u) r = r * p;
+    p = p * p;
+    n >>= 1;
+  }
+  return r;
+}
+
+// Label at a voxel of brick `bi` quantised to the brick's frame LOD; false
+// when the brick is empty or not resident (getVoxelColor, rayTrace.wgsl:160-204).
+template <typename LT>
+__device__ __forceinline__ bool brick_label(const MarchArgs<LT>& a, uint32_t bi, int lx, int ly, int lz, LT& out) {
+  const uint32_t meta = a.meta[bi];
+  if (meta & FCS_META_EMPTY) return false;
+  const uint32_t lod = meta & FCS_META_LOD_MASK;
+  if (lod == 0) {
+    out = a.pal0[bi];
+    return true;
+  }
+  if (!(meta & FCS_META_RESIDENT)) return false;
+  const int b = (int)a.brick_size;
+  const int lodSize = 1 << lod;
+  const float lodScale = (float)lodSize / (float)b;
+  const int loc[3] = {lx, ly, lz};
+  uint32_t cc[3];
+  _Pragma("unroll") for (int k = 0; k < 3; ++k) {
+    const int l = loc[k] < 0 ? 0 : (loc[k] > b - 1 ? b - 1 : loc[k]);
+    float sc = (float)l * lodScale;
+    sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
+    cc[k] = (uint32_t)sc;
+  }
+  out = a.cache[a.offs[bi] + fcs_morton3(cc[0], cc[1], cc[2])];
+  return true;
+}
+
+// sampleAlpha (rayTrace.wgsl:262-298): alpha at a world voxel
+template <typename LT>
+__device__ __forceinline__ float sample_alpha(const MarchArgs<LT>& a, int x, int y, int z) {
+  const int G = (int)(a.bricks_per_axis * a.brick_size);
+  if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) return 0.0f;
+  const int b = (int)a.brick_size;
+  const int bx = x / b, by = y / b, bz = z / b;
+  const uint32_t bi = fcs_morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+  LT lab;
+  if (!brick_label(a, bi, x - bx * b, y - by * b, z - bz * b, lab)) return 0.0f;
+  return alpha_of(tf_colour(a, lab));
+}
+
+// sampleAlphaInBrick (rayTrace.wgsl:301-318)
+template <typename LT>
+__device__ __forceinline__ float sample_alpha_in(const MarchArgs<LT>& a, uint32_t bi, const int bmin[3], int x, int y,
+                                                 int z) {
+  const int G = (int)(a.bricks_per_axis * a.brick_size);
+  if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) return 0.0f;
+  const int b = (int)a.brick_size;
+  const int lx = x - bmin[0], ly = y - bmin[1], lz = z - bmin[2];
+  if (lx < 0 || ly < 0 || lz < 0 || lx >= b || ly >= b || lz >= b) return sample_alpha(a, x, y, z);
+  LT lab;
+  if (!brick_label(a, bi, lx, ly, lz, lab)) return 0.0f;
+  return alpha_of(tf_colour(a, lab));
+}
+
+// traceShadowRay (rayTrace.wgsl:329-395)
+template <typename LT>
+__device__ float shadow_ray(const MarchArgs<LT>& a, const float start[3], const float normal[3], float accum,
+                            float transmittance) {
+  if (transmittance < 0.1f || accum < 0.2f) return 1.0f;
+  const uint32_t steps = max(24u, (uint32_t)(64.0f * transmittance));
+  float ll = a.light[0] * a.light[0];
+  ll = ll + a.light[1] * a.light[1];
+  ll = ll + a.light[2] * a.light[2];
+  const float len = sqrtf(ll);
+  float dir[3], rs[3];
+  _Pragma("unroll") for (int k = 0; k < 3; ++k) {
+    dir[k] = a.light[k] / len;
+    rs[k] = start[k] + normal[k] * 0.6f;
+  }
+  const float G = a.grid;
+  float t = 0.5f, occ = 0.0f;
+  for (uint32_t i = 0; i < steps; ++i) {
+    float p[3];
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) p[k] = rs[k] + dir[k] * t;
+    if (p[0] < 0.0f || p[1] < 0.0f || p[2] < 0.0f || p[0] >= G || p[1] >= G || p[2] >= G) break;
+    const float al = sample_alpha(a, (int)floorf(p[0]), (int)floorf(p[1]), (int)floorf(p[2]));
+    occ = occ + al * 1.0f;
+    if (occ > 0.8f) return 0.0f;
+    t = t + 1.0f;
+  }
+  const float vis = expf(-occ * 1.5f);
+  return 1.0f * (1.0f - transmittance) + vis * transmittance;
+}
+
+template <typename LT, bool RGBA>
 __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -43,6 +180,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   LT hit_label = 0;
   uint64_t hit_vox = ~0ull;
   float hit_t = -1.0f;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;  // composited colour (RGBA form)
   const int b = (int)a.brick_size;
   const int B = (int)a.bricks_per_axis;
   const float G = a.grid;
@@ -192,13 +330,76 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
                 }
                 v = lab[fcs_morton3(cc[0], cc[1], cc[2])];
               }
-              if (v != 0) {
+              if (v != 0 && hit_t < 0.0f) {
                 hit_label = v;
                 const uint64_t Gi = (uint64_t)B * (uint64_t)b;
                 hit_vox = (uint64_t)(bmin[0] + vox[0]) +
                           Gi * ((uint64_t)(bmin[1] + vox[1]) + Gi * (uint64_t)(bmin[2] + vox[2]));
                 hit_t = tcur;
-                break;
+                if (!RGBA) break;
+              }
+              if constexpr (RGBA) {
+                const uint32_t rgba = tf_colour(a, v);
+                const float al = alpha_of(rgba);
+                if (al > 0.05f) {
+                  float sr = (float)(rgba & 0xFFu) / 255.0f, sg = (float)((rgba >> 8) & 0xFFu) / 255.0f,
+                        sb = (float)((rgba >> 16) & 0xFFu) / 255.0f;
+                  if (al > 0.35f) {
+                    // rayTrace.wgsl:807-837
+                    const int vp[3] = {bmin[0] + vox[0], bmin[1] + vox[1], bmin[2] + vox[2]};
+                    float wp[3];
+                    _Pragma("unroll") for (int k = 0; k < 3; ++k) wp[k] = (float)vp[k] + 0.5f;
+                    const float xp = sample_alpha_in(a, bi, bmin, vp[0] + stride, vp[1], vp[2]);
+                    const float xn = sample_alpha_in(a, bi, bmin, vp[0] - stride, vp[1], vp[2]);
+                    const float yp = sample_alpha_in(a, bi, bmin, vp[0], vp[1] + stride, vp[2]);
+                    const float yn = sample_alpha_in(a, bi, bmin, vp[0], vp[1] - stride, vp[2]);
+                    const float zp = sample_alpha_in(a, bi, bmin, vp[0], vp[1], vp[2] + stride);
+                    const float zn = sample_alpha_in(a, bi, bmin, vp[0], vp[1], vp[2] - stride);
+                    const float g[3] = {((xp > 0.05f ? xp : al) - (xn > 0.05f ? xn : al)) * 0.5f,
+                                        ((yp > 0.05f ? yp : al) - (yn > 0.05f ? yn : al)) * 0.5f,
+                                        ((zp > 0.05f ? zp : al) - (zn > 0.05f ? zn : al)) * 0.5f};
+                    float gl = g[0] * g[0];
+                    gl = gl + g[1] * g[1];
+                    gl = gl + g[2] * g[2];
+                    const float glen = fmx(sqrtf(gl), 0.001f);
+                    float nrm[3];
+                    _Pragma("unroll") for (int k = 0; k < 3; ++k) nrm[k] = -g[k] / glen;
+                    float sf = 1.0f;
+                    if (al > 0.4f) sf = shadow_ray(a, wp, nrm, ca, 1.0f - ca);
+                    float ll = a.light[0] * a.light[0];
+                    ll = ll + a.light[1] * a.light[1];
+                    ll = ll + a.light[2] * a.light[2];
+                    const float llen = sqrtf(ll);
+                    float dl = nrm[0] * (a.light[0] / llen);
+                    dl = dl + nrm[1] * (a.light[1] / llen);
+                    dl = dl + nrm[2] * (a.light[2] / llen);
+                    const float diff = fmx(0.0f, dl);
+                    const float ds = a.diffuse * sf;
+                    const float mina = diff > 0.01f ? 0.3f : 0.1f;
+                    const float shf = fmn(fmx(a.ambient + diff * ds, mina), 1.0f);
+                    float lum = sr * 0.299f;
+                    lum = lum + sg * 0.587f;
+                    lum = lum + sb * 0.114f;
+                    const float smix = fmn(fmx(1.0f - sf, 0.0f), 1.0f) * 0.5f;
+                    const float c3[3] = {sr, sg, sb};
+                    float o3[3];
+                    _Pragma("unroll") for (int k = 0; k < 3; ++k) {
+                      const float gray = lum * 0.8f + c3[k] * 0.2f;
+                      const float base = c3[k] * (1.0f - smix) + gray * smix;
+                      o3[k] = base * shf;
+                    }
+                    sr = o3[0];
+                    sg = o3[1];
+                    sb = o3[2];
+                  }
+                  const float corr = fmn(fmx(1.0f - powi(1.0f - al, (uint32_t)stride), 0.0f), 1.0f);
+                  const float src = corr * (1.0f - ca);
+                  cr = cr + sr * src;
+                  cg = cg + sg * src;
+                  cb = cb + sb * src;
+                  ca = ca + src;
+                  if (ca > 0.95f) break;
+                }
               }
             }
             const float tt = advance3(vox[0], vox[1], vox[2], vtmax[0], vtmax[1], vtmax[2], vtdelta[0],
@@ -206,7 +407,8 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
             tcur = tt;
             if (tt > tExitBrick) break;
           }
-          if (hit_t >= 0.0f) break;
+          if (!RGBA && hit_t >= 0.0f) break;
+          if (RGBA && ca > 0.95f) break;
         }
       }
       const float tt = advance3(brick[0], brick[1], brick[2], btmax[0], btmax[1], btmax[2], btdelta[0],
@@ -218,6 +420,11 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   if (a.out_label) a.out_label[pix] = hit_label;
   if (a.out_voxel) a.out_voxel[pix] = hit_vox;
   if (a.out_t) a.out_t[pix] = hit_t;
+  if constexpr (RGBA) {
+    // rayTrace.wgsl:899-905
+    a.out_rgba[pix] = (uint32_t)(cr * 255.0f) | ((uint32_t)(cg * 255.0f) << 8) | ((uint32_t)(cb * 255.0f) << 16) |
+                      ((uint32_t)(ca * 255.0f) << 24);
+  }
 }
 
 }  // namespace
@@ -277,7 +484,12 @@ hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s) {
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {
   const dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
-  hipLaunchKernelGGL(march_kernel<LT>, grid, dim3(256), 0, s, a);
+  if (a.out_rgba) {
+    if (!a.tf_keys || !a.tf_rgba) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((march_kernel<LT, true>), grid, dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((march_kernel<LT, false>), grid, dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 

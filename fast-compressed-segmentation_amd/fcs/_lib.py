@@ -86,7 +86,13 @@ class FrameStruct(ctypes.Structure):
         ("pad", ctypes.c_uint32),
         ("decode_ms", ctypes.c_float),
         ("march_ms", ctypes.c_float),
+        ("rgba", c_u32p),
     ]
+
+
+class LightingStruct(ctypes.Structure):
+    _fields_ = [("light_angle_deg", ctypes.c_float), ("ambient", ctypes.c_float),
+                ("diffuse_strength", ctypes.c_float), ("pad", ctypes.c_float)]
 
 
 class TimingStruct(ctypes.Structure):
@@ -239,6 +245,12 @@ _sig("fcs_serve_admit", ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(ctype
      ctypes.c_uint64, c_u32p, c_u64p)
 _sig("fcs_serve_run", ctypes.c_int, ctypes.POINTER(c_vp), ctypes.c_uint32, ctypes.POINTER(c_vp), ctypes.c_uint32,
      ctypes.POINTER(ServeRequestStruct), ctypes.c_uint32, ctypes.POINTER(ServeResultStruct))
+
+
+_sig("fcs_set_lighting", ctypes.c_int, c_vp, ctypes.POINTER(LightingStruct))
+_sig("fcs_set_label_visibility", ctypes.c_int, c_vp, c_u64p, ctypes.c_uint32, c_u64p)
+_sig("fcs_transfer_function", ctypes.c_int64, c_vp, c_u64p, c_u32p, ctypes.c_uint64)
+_sig("fcs_tf_default_rgba", ctypes.c_uint32, ctypes.c_uint64)
 
 
 def check(rc, ctx=None):

@@ -5,6 +5,7 @@ import math
 import numpy as np
 
 from ._lib import (
+    LightingStruct,
     CameraStruct,
     FcsError,
     FrameStruct,
@@ -54,7 +55,9 @@ class Camera:
 
 
 class Frame:
-    def __init__(self, label, voxel, t_hit, brick_visited, brick_lod_used, bricks_decoded, decode_ms, march_ms):
+    def __init__(self, label, voxel, t_hit, brick_visited, brick_lod_used, bricks_decoded, decode_ms, march_ms,
+                 rgba=None):
+        self.rgba = rgba
         self.label = label
         self.voxel = voxel
         self.t_hit = t_hit
@@ -235,8 +238,10 @@ class Volume:
             "t_hit": np.zeros(n, np.float32) if "t_hit" in want else None,
             "vis": np.zeros(nb, np.uint32) if "brick_visited" in want else None,
             "used": np.zeros(nb, np.uint8),
+            "rgba": np.zeros(n, np.uint32) if "rgba" in want else None,
         }
         f = FrameStruct()
+        f.rgba = _ptr(fr["rgba"], c_u32p) if fr["rgba"] is not None else None
         f.label = fr["label"].ctypes.data if fr["label"] is not None else None
         f.voxel = _ptr(fr["voxel"], c_u64p) if fr["voxel"] is not None else None
         f.t_hit = fr["t_hit"].ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fr["t_hit"] is not None else None
@@ -247,28 +252,43 @@ class Volume:
     @staticmethod
     def _frame(fr, f):
         return Frame(fr["label"], fr["voxel"], fr["t_hit"], fr["vis"], fr["used"], f.bricks_decoded, f.decode_ms,
-                     f.march_ms)
+                     f.march_ms, fr["rgba"])
 
     def render(self, cam, w, h, brick_lod=None, want=("label", "voxel", "t_hit", "brick_visited")):
-        n = w * h
-        nb = self.info["num_bricks"]
-        label = np.zeros(n, self.label_dtype) if "label" in want else None
-        voxel = np.zeros(n, np.uint64) if "voxel" in want else None
-        t_hit = np.zeros(n, np.float32) if "t_hit" in want else None
-        vis = np.zeros(nb, np.uint32) if "brick_visited" in want else None
-        used = np.zeros(nb, np.uint8)
-        f = FrameStruct()
-        f.label = label.ctypes.data if label is not None else None
-        f.voxel = _ptr(voxel, c_u64p) if voxel is not None else None
-        f.t_hit = t_hit.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if t_hit is not None else None
-        f.brick_visited = _ptr(vis, c_u32p) if vis is not None else None
-        f.brick_lod_used = _ptr(used, c_u8p)
+        """fcs_render; `want` names the outputs to copy back ("label", "voxel",
+        "t_hit", "brick_visited", "rgba": the shaded composite)."""
+        fr, f = self._frame_buffers(w, h, want)
         lp = None
         if brick_lod is not None:
             bl = np.ascontiguousarray(brick_lod, dtype=np.uint8)
             lp = _ptr(bl, c_u8p)
         check(lib.fcs_render(self.h, ctypes.byref(cam.s), lp, w, h, ctypes.byref(f)), self.ctx.h)
-        return Frame(label, voxel, t_hit, vis, used, f.bricks_decoded, f.decode_ms, f.march_ms)
+        return self._frame(fr, f)
+
+    def set_lighting(self, light_angle_deg=25.0, ambient=0.5, diffuse_strength=0.8):
+        """updateLightingOptions (webgpuSetup.js:186-222)."""
+        ls = LightingStruct(light_angle_deg, ambient, diffuse_strength, 0.0)
+        check(lib.fcs_set_lighting(self.h, ctypes.byref(ls)), self.ctx.h)
+
+    def set_label_visibility(self, hidden=(), highlight=None):
+        """applyLabelVisibility + applyLabelHover (webgpuSetup.js:827-882)."""
+        hs = np.ascontiguousarray(np.array(list(hidden), dtype=np.uint64))
+        hl = ctypes.c_uint64(int(highlight)) if highlight is not None else None
+        check(lib.fcs_set_label_visibility(self.h, _ptr(hs, c_u64p) if hs.size else None, hs.size,
+                                           ctypes.byref(hl) if hl is not None else None), self.ctx.h)
+
+    def transfer_function(self):
+        """getDatasetLabels + the current colours: (labels ascending, RGBA8)."""
+        n = check(lib.fcs_transfer_function(self.h, None, None, 0), self.ctx.h)
+        labels = np.zeros(n, np.uint64)
+        rgba = np.zeros(n, np.uint32)
+        check(lib.fcs_transfer_function(self.h, _ptr(labels, c_u64p), _ptr(rgba, c_u32p), n), self.ctx.h)
+        return labels, rgba
+
+    def lighting_direction(self, light_angle_deg=25.0):
+        """The light direction of the lighting uniform, float32."""
+        a = light_angle_deg * math.pi / 180.0
+        return np.array([math.sin(a), 0.6, -math.cos(a)], np.float32)
 
 
 def load_dataset_from_array_buffer(ctx, array_buffer, brick_size_override=0):

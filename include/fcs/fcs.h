@@ -105,7 +105,22 @@ typedef struct {
   uint32_t pad;
   float decode_ms;         /* device time of this frame's decode (hipEvent) */
   float march_ms;          /* device time of this frame's march (hipEvent) */
+  /* caller-set, nullable: w*h RGBA8 pixels (R in bits 0-7, A in 24-31) of the
+   * shaded front-to-back composite of rayTrace.wgsl:565-906 (transfer
+   * function, LOD gradient, diffuse light, shadow rays, early end at alpha
+   * 0.95). When set, the march runs in its compositing form; the first-hit
+   * outputs are unchanged. */
+  uint32_t* rgba;
 } fcs_frame;
+
+/* Lighting of shaded frames (webgpuSetup.js:21-31, 186-222): light direction
+ * (sin(angle), 0.6, -cos(angle)), ambient term, diffuse strength. */
+typedef struct {
+  float light_angle_deg;  /* default 25 */
+  float ambient;          /* default 0.5 */
+  float diffuse_strength; /* default 0.8 */
+  float pad;
+} fcs_lighting;
 
 typedef struct {
   float rans_ms;          /* last fcs_decode_rans */
@@ -231,6 +246,23 @@ int fcs_camera_features(const fcs_volume* vol, const fcs_camera* cam, float* out
 
 /* Orbit pose around the volume centre (orbitController.js:100-123) with the
  * basis of camera.js:61-77, fov in radians. */
+/* Shading controls. updateLightingOptions (webgpuSetup.js:186-222). */
+int fcs_set_lighting(fcs_volume* vol, const fcs_lighting* lighting);
+/* applyLabelVisibility + applyLabelHover (webgpuSetup.js:827-882): labels in
+ * `hidden` become transparent; `highlight` (nullable) is brightened x1.8
+ * (clamped at 255); each call starts again from the default colours. */
+int fcs_set_label_visibility(fcs_volume* vol, const uint64_t* hidden, uint32_t n_hidden, const uint64_t* highlight);
+/* getDatasetLabels / buildTransferFunctionBuffers (webgpuSetup.js:57-59,
+ * 389-458): the transfer function's labels (every distinct non-zero palette
+ * label, ascending) and their current RGBA8 colours; writes min(cap, n)
+ * entries (either pointer nullable) and returns n. */
+int64_t fcs_transfer_function(fcs_volume* vol, uint64_t* labels, uint32_t* rgba, uint64_t cap);
+/* Default RGBA8 of transfer function entry `index` (webgpuSetup.js:410-429,
+ * hslToRgb 490-515, JavaScript double arithmetic): hue index*137.5 mod 360,
+ * saturation 0.55 + (index mod 3)*0.08, lightness 0.35 + (index mod 4)*0.05,
+ * opacity 0.25 + (index mod 5)*0.08. */
+uint32_t fcs_tf_default_rgba(uint64_t index);
+
 int fcs_camera_orbit(const fcs_volume* vol, double yaw, double pitch, double radius, double fov,
                      fcs_camera* out);
 

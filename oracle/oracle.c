@@ -336,180 +336,416 @@ static float adv(int v[3], float tmax[3], const float tdelta[3], const int step[
 
 static float bias(float t) { return fmaxf_(t * 1e-6f, 1e-5f); }
 
-int or_render(uint32_t B, uint32_t b, const uint8_t* brick_empty, const uint8_t* brick_lod,
-              const uint64_t* const* brick_labels, const uint64_t* palette0, const or_camera* cam,
-              uint32_t w, uint32_t h, uint64_t* out_label, uint64_t* out_voxel, float* out_t,
-              uint32_t* visited) {
+/* Scene of one frame; the compositing fields are used by or_render_rgba. */
+typedef struct {
+  uint32_t B, b;
+  const uint8_t* empty;
+  const uint8_t* lod;
+  const uint64_t* const* labels;
+  const uint64_t* pal0;
+  int rgba;
+  const uint64_t* tf_labels;
+  const uint32_t* tf_rgba;
+  uint64_t tf_n;
+  float light[3], ambient, diffuse;
+} scene_t;
+
+/* ------------------------------------------------------------ shading --- */
+/* decode.wgsl:135-172: transfer function entry, else a hash colour */
+static uint32_t hash_colour(uint64_t label) {
+  uint32_t lo = (uint32_t)label, hi = (uint32_t)(label >> 32);
+  lo = lo ^ hi;
+  lo = lo ^ (lo >> 16);
+  lo = lo * 0x7feb352du;
+  lo = lo ^ (lo >> 15);
+  hi = hi ^ lo;
+  hi = hi ^ (hi >> 16);
+  hi = hi * 0x85ebca6bu;
+  hi = hi ^ (hi >> 13);
+  return (lo & 0xFFu) | (((lo >> 8) & 0xFFu) << 8) | (((hi >> 8) & 0xFFu) << 16) | (0x40u << 24);
+}
+
+static uint32_t tf_colour(const scene_t* sc, uint64_t label) {
+  if (label == 0) return 0u;
+  uint64_t lo = 0, hi = sc->tf_n;
+  while (lo < hi) { /* binary search in the ascending labels */
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (sc->tf_labels[mid] < label) lo = mid + 1; else hi = mid;
+  }
+  if (lo < sc->tf_n && sc->tf_labels[lo] == label) return sc->tf_rgba[lo];
+  return hash_colour(label);
+}
+
+static float alpha_of(uint32_t rgba) { return (float)((rgba >> 24) & 0xFFu) / 255.0f; }
+
+static float powi(float x, uint32_t n) { /* repeated squaring */
+  float r = 1.0f, p = x;
+  while (n) {
+    if (n & 1u) r = r * p;
+    p = p * p;
+    n >>= 1;
+  }
+  return r;
+}
+
+/* getVoxelColor's lookup: label of a voxel of brick bi at its frame LOD;
+ * 0 when the brick is empty or has no labels (not resident) */
+static int brick_label(const scene_t* sc, uint32_t bi, int lx, int ly, int lz, uint64_t* out) {
+  if (sc->empty[bi]) return 0;
+  const uint32_t lod = sc->lod[bi];
+  if (lod == 0) { *out = sc->pal0[bi]; return 1; }
+  if (!sc->labels[bi]) return 0;
+  const int b = (int)sc->b, lodSize = 1 << lod;
+  const float lodScale = (float)lodSize / (float)b;
+  const int loc[3] = {lx, ly, lz};
+  uint32_t cc[3];
+  for (int k = 0; k < 3; ++k) {
+    const int l = loc[k] < 0 ? 0 : (loc[k] > b - 1 ? b - 1 : loc[k]);
+    float s = (float)l * lodScale;
+    s = s < 0.0f ? 0.0f : (s > (float)(lodSize - 1) ? (float)(lodSize - 1) : s);
+    cc[k] = (uint32_t)s;
+  }
+  *out = sc->labels[bi][morton3(cc[0], cc[1], cc[2])];
+  return 1;
+}
+
+/* sampleAlpha (rayTrace.wgsl:262-298) */
+static float sample_alpha(const scene_t* sc, int x, int y, int z) {
+  const int G = (int)(sc->B * sc->b), b = (int)sc->b;
+  if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) return 0.0f;
+  const int bx = x / b, by = y / b, bz = z / b;
+  uint64_t lab;
+  if (!brick_label(sc, morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz), x - bx * b, y - by * b, z - bz * b, &lab))
+    return 0.0f;
+  return alpha_of(tf_colour(sc, lab));
+}
+
+/* sampleAlphaInBrick (rayTrace.wgsl:301-318) */
+static float sample_alpha_in(const scene_t* sc, uint32_t bi, const int bmin[3], int x, int y, int z) {
+  const int G = (int)(sc->B * sc->b), b = (int)sc->b;
+  if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) return 0.0f;
+  const int lx = x - bmin[0], ly = y - bmin[1], lz = z - bmin[2];
+  if (lx < 0 || ly < 0 || lz < 0 || lx >= b || ly >= b || lz >= b) return sample_alpha(sc, x, y, z);
+  uint64_t lab;
+  if (!brick_label(sc, bi, lx, ly, lz, &lab)) return 0.0f;
+  return alpha_of(tf_colour(sc, lab));
+}
+
+/* traceShadowRay (rayTrace.wgsl:329-395) */
+static float shadow_ray(const scene_t* sc, const float start[3], const float normal[3], float accum,
+                        float transmittance) {
+  if (transmittance < 0.1f || accum < 0.2f) return 1.0f;
+  const uint32_t s64 = (uint32_t)(64.0f * transmittance);
+  const uint32_t steps = s64 > 24u ? s64 : 24u;
+  float ll = sc->light[0] * sc->light[0];
+  ll = ll + sc->light[1] * sc->light[1];
+  ll = ll + sc->light[2] * sc->light[2];
+  const float len = sqrtf(ll);
+  float dir[3], rs[3];
+  for (int k = 0; k < 3; ++k) {
+    dir[k] = sc->light[k] / len;
+    rs[k] = start[k] + normal[k] * 0.6f;
+  }
+  const float G = (float)(sc->B * sc->b);
+  float t = 0.5f, occ = 0.0f;
+  for (uint32_t i = 0; i < steps; ++i) {
+    float p[3];
+    for (int k = 0; k < 3; ++k) p[k] = rs[k] + dir[k] * t;
+    if (p[0] < 0.0f || p[1] < 0.0f || p[2] < 0.0f || p[0] >= G || p[1] >= G || p[2] >= G) break;
+    const float al = sample_alpha(sc, (int)floorf(p[0]), (int)floorf(p[1]), (int)floorf(p[2]));
+    occ = occ + al * 1.0f;
+    if (occ > 0.8f) return 0.0f;
+    t = t + 1.0f;
+  }
+  const float vis = expf(-occ * 1.5f);
+  return 1.0f * (1.0f - transmittance) + vis * transmittance;
+}
+
+/* rayTrace.wgsl:801-837: colour of one sample of alpha al at voxel vp */
+static void shade_sample(const scene_t* sc, uint32_t bi, const int bmin[3], const int vp[3], int stride, float al,
+                         float ca, float* sr, float* sg, float* sb) {
+  float wp[3];
+  for (int k = 0; k < 3; ++k) wp[k] = (float)vp[k] + 0.5f;
+  const float xp = sample_alpha_in(sc, bi, bmin, vp[0] + stride, vp[1], vp[2]);
+  const float xn = sample_alpha_in(sc, bi, bmin, vp[0] - stride, vp[1], vp[2]);
+  const float yp = sample_alpha_in(sc, bi, bmin, vp[0], vp[1] + stride, vp[2]);
+  const float yn = sample_alpha_in(sc, bi, bmin, vp[0], vp[1] - stride, vp[2]);
+  const float zp = sample_alpha_in(sc, bi, bmin, vp[0], vp[1], vp[2] + stride);
+  const float zn = sample_alpha_in(sc, bi, bmin, vp[0], vp[1], vp[2] - stride);
+  const float g[3] = {((xp > 0.05f ? xp : al) - (xn > 0.05f ? xn : al)) * 0.5f,
+                      ((yp > 0.05f ? yp : al) - (yn > 0.05f ? yn : al)) * 0.5f,
+                      ((zp > 0.05f ? zp : al) - (zn > 0.05f ? zn : al)) * 0.5f};
+  float gl = g[0] * g[0];
+  gl = gl + g[1] * g[1];
+  gl = gl + g[2] * g[2];
+  const float glen = fmaxf_(sqrtf(gl), 0.001f);
+  float nrm[3];
+  for (int k = 0; k < 3; ++k) nrm[k] = -g[k] / glen;
+  float sf = 1.0f;
+  if (al > 0.4f) sf = shadow_ray(sc, wp, nrm, ca, 1.0f - ca);
+  float ll = sc->light[0] * sc->light[0];
+  ll = ll + sc->light[1] * sc->light[1];
+  ll = ll + sc->light[2] * sc->light[2];
+  const float llen = sqrtf(ll);
+  float dl = nrm[0] * (sc->light[0] / llen);
+  dl = dl + nrm[1] * (sc->light[1] / llen);
+  dl = dl + nrm[2] * (sc->light[2] / llen);
+  const float diff = fmaxf_(0.0f, dl);
+  const float ds = sc->diffuse * sf;
+  const float mina = diff > 0.01f ? 0.3f : 0.1f;
+  const float shf = fminf_(fmaxf_(sc->ambient + diff * ds, mina), 1.0f);
+  float lum = *sr * 0.299f;
+  lum = lum + *sg * 0.587f;
+  lum = lum + *sb * 0.114f;
+  const float smix = fminf_(fmaxf_(1.0f - sf, 0.0f), 1.0f) * 0.5f;
+  float* c3[3] = {sr, sg, sb};
+  for (int k = 0; k < 3; ++k) {
+    const float gray = lum * 0.8f + *c3[k] * 0.2f;
+    const float base = *c3[k] * (1.0f - smix) + gray * smix;
+    *c3[k] = base * shf;
+  }
+}
+
+/* One pixel: the first hit (label != 0) and, in the compositing form, the
+ * shaded front-to-back composite (rayTrace.wgsl:565-906). */
+static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                        uint64_t* out_label, uint64_t* out_voxel, float* out_t, uint32_t* out_rgba,
+                        uint32_t* visited) {
+  const uint32_t B = sc->B, b = sc->b;
   const float Wf = (float)w, Hf = (float)h;
   const float aspect = (float)((double)w / (double)h);
   const float G = (float)(B * b);
   const uint64_t Gi = (uint64_t)B * b;
-  for (uint32_t py = 0; py < h; ++py) {
-    for (uint32_t px = 0; px < w; ++px) {
-      const size_t pix = (size_t)py * w + px;
-      uint64_t hit_label = 0, hit_vox = UINT64_MAX;
-      float hit_t = -1.0f;
-      /* rayTrace.wgsl:597-619 */
-      float ndc_x = ((float)px + 0.5f) / Wf;
-      ndc_x = ndc_x * 2.0f;
-      ndc_x = ndc_x - 1.0f;
-      float ndc_y = ((float)py + 0.5f) / Hf;
-      ndc_y = ndc_y * 2.0f;
-      ndc_y = 1.0f - ndc_y;
-      ray_t r;
-      if (cam->ortho_half_height > 0.0f) {
-        for (int a = 0; a < 3; ++a) {
-          float ox = cam->right[a] * ndc_x;
-          ox = ox * aspect;
-          ox = ox * cam->ortho_half_height;
-          float oy = cam->up[a] * ndc_y;
-          oy = oy * cam->ortho_half_height;
-          r.o[a] = (cam->position[a] + ox) + oy;
-          r.d[a] = cam->forward[a];
-        }
-      } else {
-        float v[3];
-        for (int a = 0; a < 3; ++a) {
-          float rt = cam->right[a] * ndc_x;
-          rt = rt * aspect;
-          rt = rt * cam->tan_half_fov;
-          float ut = cam->up[a] * ndc_y;
-          ut = ut * cam->tan_half_fov;
-          v[a] = (cam->forward[a] + rt) + ut;
-          r.o[a] = cam->position[a];
-        }
-        float dd = v[0] * v[0];
-        dd = dd + v[1] * v[1];
-        dd = dd + v[2] * v[2];
-        const float len = sqrtf(dd);
-        for (int a = 0; a < 3; ++a) r.d[a] = v[a] / len;
-      }
+  uint64_t hit_label = 0, hit_vox = UINT64_MAX;
+  float hit_t = -1.0f;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
+  /* rayTrace.wgsl:597-619 */
+  float ndc_x = ((float)px + 0.5f) / Wf;
+  ndc_x = ndc_x * 2.0f;
+  ndc_x = ndc_x - 1.0f;
+  float ndc_y = ((float)py + 0.5f) / Hf;
+  ndc_y = ndc_y * 2.0f;
+  ndc_y = 1.0f - ndc_y;
+  ray_t r;
+  if (cam->ortho_half_height > 0.0f) {
+    for (int a = 0; a < 3; ++a) {
+      float ox = cam->right[a] * ndc_x;
+      ox = ox * aspect;
+      ox = ox * cam->ortho_half_height;
+      float oy = cam->up[a] * ndc_y;
+      oy = oy * cam->ortho_half_height;
+      r.o[a] = (cam->position[a] + ox) + oy;
+      r.d[a] = cam->forward[a];
+    }
+  } else {
+    float v[3];
+    for (int a = 0; a < 3; ++a) {
+      float rt = cam->right[a] * ndc_x;
+      rt = rt * aspect;
+      rt = rt * cam->tan_half_fov;
+      float ut = cam->up[a] * ndc_y;
+      ut = ut * cam->tan_half_fov;
+      v[a] = (cam->forward[a] + rt) + ut;
+      r.o[a] = cam->position[a];
+    }
+    float dd = v[0] * v[0];
+    dd = dd + v[1] * v[1];
+    dd = dd + v[2] * v[2];
+    const float len = sqrtf(dd);
+    for (int a = 0; a < 3; ++a) r.d[a] = v[a] / len;
+  }
+  for (int a = 0; a < 3; ++a) {
+    r.zero[a] = r.d[a] == 0.0f;
+    const float s = r.d[a] > 0.0f ? 1.0f : (r.d[a] < 0.0f ? -1.0f : 0.0f);
+    r.inv[a] = r.zero[a] ? INFINITY : s / fmaxf_(fabsf(r.d[a]), 1e-6f);
+  }
+  float tEnter, tExit;
+  if (!intersect_volume(&r, G, &tEnter, &tExit)) goto store;
+  {
+    /* initBrickDDA (rayTrace.wgsl:519-545) */
+    int step[3], brick[3];
+    float btmax[3], btdelta[3];
+    {
+      const float ts = tEnter + bias(tEnter);
       for (int a = 0; a < 3; ++a) {
-        r.zero[a] = r.d[a] == 0.0f;
-        const float s = r.d[a] > 0.0f ? 1.0f : (r.d[a] < 0.0f ? -1.0f : 0.0f);
-        r.inv[a] = r.zero[a] ? INFINITY : s / fmaxf_(fabsf(r.d[a]), 1e-6f);
+        const float pos = r.o[a] + r.d[a] * ts;
+        brick[a] = (int)floorf(pos) / (int)b;
+        step[a] = r.d[a] > 0.0f ? 1 : (r.d[a] < 0.0f ? -1 : 0);
+        const float bf = (float)brick[a] * (float)b;
+        const float nb = r.d[a] > 0.0f ? bf + (float)b : bf;
+        btmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+        btdelta[a] = r.zero[a] ? INFINITY : fabsf((float)b * r.inv[a]);
       }
-      float tEnter, tExit;
-      if (!intersect_volume(&r, G, &tEnter, &tExit)) goto store;
-      {
-        /* initBrickDDA (rayTrace.wgsl:519-545) */
-        int step[3], brick[3];
-        float btmax[3], btdelta[3];
-        {
-          const float ts = tEnter + bias(tEnter);
-          for (int a = 0; a < 3; ++a) {
-            const float pos = r.o[a] + r.d[a] * ts;
-            brick[a] = (int)floorf(pos) / (int)b;
-            step[a] = r.d[a] > 0.0f ? 1 : (r.d[a] < 0.0f ? -1 : 0);
-            const float bf = (float)brick[a] * (float)b;
-            const float nb = r.d[a] > 0.0f ? bf + (float)b : bf;
-            btmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
-            btdelta[a] = r.zero[a] ? INFINITY : fabsf((float)b * r.inv[a]);
-          }
-        }
-        int be[3], bx[3];
-        uint32_t maxb = 6;
+    }
+    int be[3], bx[3];
+    uint32_t maxb = 6;
+    for (int a = 0; a < 3; ++a) {
+      be[a] = (int)(r.o[a] + r.d[a] * tEnter) / (int)b;
+      bx[a] = (int)(r.o[a] + r.d[a] * tExit) / (int)b;
+      maxb += (uint32_t)abs(bx[a] - be[a]);
+    }
+    float brickEntryT = tEnter;
+    for (uint32_t it = 0; it < maxb; ++it) {
+      if (brick[0] < 0 || brick[1] < 0 || brick[2] < 0 || brick[0] >= (int)B || brick[1] >= (int)B ||
+          brick[2] >= (int)B)
+        break;
+      const uint32_t bi = morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
+      if (!sc->empty[bi]) {
+        if (visited) visited[bi] = 1;
+        const int bmin[3] = {brick[0] * (int)b, brick[1] * (int)b, brick[2] * (int)b};
+        const float tEnterBrick = fmaxf_(brickEntryT, tEnter);
+        float tExitBrick = fminf_(fminf_(btmax[0], btmax[1]), btmax[2]);
+        tExitBrick = fmaxf_(tEnterBrick, tExitBrick - bias(tExitBrick));
+        /* initDDA (rayTrace.wgsl:473-499) */
+        int vox[3], vstep[3];
+        float vtmax[3], vtdelta[3];
+        const float ts = tEnterBrick + bias(tEnterBrick);
         for (int a = 0; a < 3; ++a) {
-          be[a] = (int)(r.o[a] + r.d[a] * tEnter) / (int)b;
-          bx[a] = (int)(r.o[a] + r.d[a] * tExit) / (int)b;
-          maxb += (uint32_t)abs(bx[a] - be[a]);
+          const float pos = r.o[a] + r.d[a] * ts;
+          vox[a] = (int)floorf(pos);
+          vstep[a] = step[a];
+          const float nb = r.d[a] > 0.0f ? (float)(vox[a] + 1) : (float)vox[a];
+          vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+          vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]);
         }
-        float brickEntryT = tEnter;
-        for (uint32_t it = 0; it < maxb; ++it) {
-          if (brick[0] < 0 || brick[1] < 0 || brick[2] < 0 || brick[0] >= (int)B || brick[1] >= (int)B ||
-              brick[2] >= (int)B)
-            break;
-          const uint32_t bi = morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
-          if (!brick_empty[bi]) {
-            if (visited) visited[bi] = 1;
-            const int bmin[3] = {brick[0] * (int)b, brick[1] * (int)b, brick[2] * (int)b};
-            const float tEnterBrick = fmaxf_(brickEntryT, tEnter);
-            float tExitBrick = fminf_(fminf_(btmax[0], btmax[1]), btmax[2]);
-            tExitBrick = fmaxf_(tEnterBrick, tExitBrick - bias(tExitBrick));
-            /* initDDA (rayTrace.wgsl:473-499) */
-            int vox[3], vstep[3];
-            float vtmax[3], vtdelta[3];
-            const float ts = tEnterBrick + bias(tEnterBrick);
-            for (int a = 0; a < 3; ++a) {
-              const float pos = r.o[a] + r.d[a] * ts;
-              vox[a] = (int)floorf(pos);
-              vstep[a] = step[a];
-              const float nb = r.d[a] > 0.0f ? (float)(vox[a] + 1) : (float)vox[a];
-              vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
-              vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]);
-            }
-            const uint32_t lod = brick_lod[bi];
-            const int lodSize = 1 << lod;
-            const int stride = (int)b / lodSize;
-            for (int a = 0; a < 3; ++a) vox[a] -= bmin[a];
-            if (stride > 1 && OR_MUTANT != 4) {
-              for (int a = 0; a < 3; ++a) {
-                int lv = (vox[a] / stride) * stride;
-                lv = lv < 0 ? 0 : (lv > (int)b - 1 ? (int)b - 1 : lv);
-                vox[a] = lv;
-                /* the reference keeps a unit voxel step here (rayTrace.wgsl:721-744)
-                 * while tDelta covers `stride` voxels, so coarse cells are sampled
-                 * out of place; the cell walk steps by the stride */
-                vstep[a] = step[a] * stride;
-                vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]) * (float)stride;
-                const float lw = (float)bmin[a] + (float)vox[a];
-                const float nb = r.d[a] > 0.0f ? lw + (float)stride : lw;
-                vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
-              }
+        const uint32_t lod = sc->lod[bi];
+        const int lodSize = 1 << lod;
+        const int stride = (int)b / lodSize;
+        for (int a = 0; a < 3; ++a) vox[a] -= bmin[a];
+        if (stride > 1 && OR_MUTANT != 4) {
+          for (int a = 0; a < 3; ++a) {
+            int lv = (vox[a] / stride) * stride;
+            lv = lv < 0 ? 0 : (lv > (int)b - 1 ? (int)b - 1 : lv);
+            vox[a] = lv;
+            /* the reference keeps a unit voxel step here (rayTrace.wgsl:721-744)
+             * while tDelta covers `stride` voxels, so coarse cells are sampled
+             * out of place; the cell walk steps by the stride */
+            vstep[a] = step[a] * stride;
+            vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]) * (float)stride;
+            const float lw = (float)bmin[a] + (float)vox[a];
+            const float nb = r.d[a] > 0.0f ? lw + (float)stride : lw;
+            vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
+          }
+        } else {
+          for (int a = 0; a < 3; ++a) vox[a] = vox[a] < 0 ? 0 : (vox[a] > (int)b - 1 ? (int)b - 1 : vox[a]);
+        }
+        uint32_t maxs = 128;
+        for (int a = 0; a < 3; ++a) {
+          const float ve = r.o[a] + r.d[a] * tEnterBrick;
+          const float vx = r.o[a] + r.d[a] * tExitBrick;
+          const float le = (ve - (float)bmin[a]) / (float)stride;
+          const float lx = (vx - (float)bmin[a]) / (float)stride;
+          maxs += (uint32_t)abs((int)ceilf(lx) - (int)floorf(le));
+        }
+        float tcur = tEnterBrick;
+        const float lodScale = (float)lodSize / (float)b;
+        for (uint32_t s = 0; s < maxs; ++s) {
+          const int inb = vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < (int)b && vox[1] < (int)b &&
+                          vox[2] < (int)b;
+          if (inb) {
+            /* getVoxelColor (rayTrace.wgsl:160-204); LOD 0 = palette[0] */
+            uint64_t lab;
+            if (lod == 0) {
+              lab = sc->pal0[bi];
             } else {
-              for (int a = 0; a < 3; ++a) vox[a] = vox[a] < 0 ? 0 : (vox[a] > (int)b - 1 ? (int)b - 1 : vox[a]);
-            }
-            uint32_t maxs = 128;
-            for (int a = 0; a < 3; ++a) {
-              const float ve = r.o[a] + r.d[a] * tEnterBrick;
-              const float vx = r.o[a] + r.d[a] * tExitBrick;
-              const float le = (ve - (float)bmin[a]) / (float)stride;
-              const float lx = (vx - (float)bmin[a]) / (float)stride;
-              maxs += (uint32_t)abs((int)ceilf(lx) - (int)floorf(le));
-            }
-            float tcur = tEnterBrick;
-            const float lodScale = (float)lodSize / (float)b;
-            for (uint32_t s = 0; s < maxs; ++s) {
-              const int inb = vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < (int)b && vox[1] < (int)b &&
-                              vox[2] < (int)b;
-              if (inb) {
-                /* getVoxelColor (rayTrace.wgsl:160-204); LOD 0 = palette[0] */
-                uint64_t lab;
-                if (lod == 0) {
-                  lab = palette0[bi];
-                } else {
-                  uint32_t cc[3];
-                  for (int a = 0; a < 3; ++a) {
-                    float sc = (float)vox[a] * lodScale;
-                    sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
-                    cc[a] = (uint32_t)sc;
-                  }
-                  lab = brick_labels[bi][morton3(cc[0], cc[1], cc[2])];
-                }
-                if (lab != 0) {
-                  hit_label = lab;
-                  hit_vox = (uint64_t)(bmin[0] + vox[0]) +
-                            Gi * ((uint64_t)(bmin[1] + vox[1]) + Gi * (uint64_t)(bmin[2] + vox[2]));
-                  hit_t = tcur;
-                  goto store;
-                }
+              uint32_t cc[3];
+              for (int a = 0; a < 3; ++a) {
+                float scl = (float)vox[a] * lodScale;
+                scl = scl < 0.0f ? 0.0f : (scl > (float)(lodSize - 1) ? (float)(lodSize - 1) : scl);
+                cc[a] = (uint32_t)scl;
               }
-              const float t = adv(vox, vtmax, vtdelta, vstep);
-              tcur = t;
-              if (t > tExitBrick) break;
+              lab = sc->labels[bi][morton3(cc[0], cc[1], cc[2])];
+            }
+            if (lab != 0 && hit_t < 0.0f) {
+              hit_label = lab;
+              hit_vox = (uint64_t)(bmin[0] + vox[0]) +
+                        Gi * ((uint64_t)(bmin[1] + vox[1]) + Gi * (uint64_t)(bmin[2] + vox[2]));
+              hit_t = tcur;
+              if (!sc->rgba) goto store;
+            }
+            if (sc->rgba) {
+              const uint32_t rgba = tf_colour(sc, lab);
+              const float al = alpha_of(rgba);
+              if (al > 0.05f) {
+                float sr = (float)(rgba & 0xFFu) / 255.0f, sg = (float)((rgba >> 8) & 0xFFu) / 255.0f,
+                      sb = (float)((rgba >> 16) & 0xFFu) / 255.0f;
+                if (al > 0.35f) {
+                  const int vp[3] = {bmin[0] + vox[0], bmin[1] + vox[1], bmin[2] + vox[2]};
+                  shade_sample(sc, bi, bmin, vp, stride, al, ca, &sr, &sg, &sb);
+                }
+                const float corr = fminf_(fmaxf_(1.0f - powi(1.0f - al, (uint32_t)(OR_MUTANT == 6 ? 1 : stride)), 0.0f), 1.0f);
+                const float src = corr * (1.0f - ca);
+                cr = cr + sr * src;
+                cg = cg + sg * src;
+                cb = cb + sb * src;
+                ca = ca + src;
+                if (ca > 0.95f) break;
+              }
             }
           }
-          const float t = adv(brick, btmax, btdelta, step);
-          brickEntryT = t + bias(t);
-          if (t > tExit) break;
+          const float t = adv(vox, vtmax, vtdelta, vstep);
+          tcur = t;
+          if (t > tExitBrick) break;
         }
+        if (sc->rgba && ca > 0.95f) break;
       }
-    store:
-      if (out_label) out_label[pix] = hit_label;
-      if (out_voxel) out_voxel[pix] = hit_vox;
-      if (out_t) out_t[pix] = hit_t;
+      const float t = adv(brick, btmax, btdelta, step);
+      brickEntryT = t + bias(t);
+      if (t > tExit) break;
     }
   }
+store:
+  if (out_label) *out_label = hit_label;
+  if (out_voxel) *out_voxel = hit_vox;
+  if (out_t) *out_t = hit_t;
+  if (out_rgba)
+    *out_rgba = (uint32_t)(cr * 255.0f) | ((uint32_t)(cg * 255.0f) << 8) | ((uint32_t)(cb * 255.0f) << 16) |
+                ((uint32_t)(ca * 255.0f) << 24);
+}
+
+int or_render(uint32_t B, uint32_t b, const uint8_t* brick_empty, const uint8_t* brick_lod,
+              const uint64_t* const* brick_labels, const uint64_t* palette0, const or_camera* cam,
+              uint32_t w, uint32_t h, uint64_t* out_label, uint64_t* out_voxel, float* out_t,
+              uint32_t* visited) {
+  scene_t sc;
+  memset(&sc, 0, sizeof sc);
+  sc.B = B;
+  sc.b = b;
+  sc.empty = brick_empty;
+  sc.lod = brick_lod;
+  sc.labels = brick_labels;
+  sc.pal0 = palette0;
+  for (uint32_t py = 0; py < h; ++py)
+    for (uint32_t px = 0; px < w; ++px) {
+      const size_t pix = (size_t)py * w + px;
+      march_pixel(&sc, cam, w, h, px, py, out_label ? out_label + pix : NULL, out_voxel ? out_voxel + pix : NULL,
+                  out_t ? out_t + pix : NULL, NULL, visited);
+    }
+  return 0;
+}
+
+int or_render_rgba(uint32_t B, uint32_t b, const uint8_t* brick_empty, const uint8_t* brick_lod,
+                   const uint64_t* const* brick_labels, const uint64_t* palette0, const or_camera* cam,
+                   uint32_t w, uint32_t h, const uint64_t* tf_labels, const uint32_t* tf_rgba, uint64_t tf_n,
+                   const float light[3], float ambient, float diffuse, uint32_t* out_rgba) {
+  scene_t sc;
+  memset(&sc, 0, sizeof sc);
+  sc.B = B;
+  sc.b = b;
+  sc.empty = brick_empty;
+  sc.lod = brick_lod;
+  sc.labels = brick_labels;
+  sc.pal0 = palette0;
+  sc.rgba = 1;
+  sc.tf_labels = tf_labels;
+  sc.tf_rgba = tf_rgba;
+  sc.tf_n = tf_n;
+  for (int k = 0; k < 3; ++k) sc.light[k] = light[k];
+  sc.ambient = ambient;
+  sc.diffuse = diffuse;
+  for (uint32_t py = 0; py < h; ++py)
+    for (uint32_t px = 0; px < w; ++px)
+      march_pixel(&sc, cam, w, h, px, py, NULL, NULL, NULL, out_rgba + (size_t)py * w + px, NULL);
   return 0;
 }
 

@@ -17,6 +17,11 @@
  *                      symmetric/axis-aligned cases in the tests (parity
  *                      partially pinned).
  *   or_brick_lod       exact thresholds of camera.js:103-137 in tests.
+ *   or_render_rgba     closed forms: a one-label volume seen along +z
+ *                      composites to 1-(1-a)^n in the label's colour; a
+ *                      planar interface gives the analytic normal and
+ *                      diffuse term; hidden labels are transparent; a
+ *                      planted mistake (opacity correction dropped) fails.
  */
 #ifndef FCS_ORACLE_H
 #define FCS_ORACLE_H
@@ -73,6 +78,18 @@ int or_render(uint32_t bricks_per_axis, uint32_t brick_size, const uint8_t* bric
               const uint8_t* brick_lod, const uint64_t* const* brick_labels,
               const uint64_t* palette0, const or_camera* cam, uint32_t w, uint32_t h,
               uint64_t* out_label, uint64_t* out_voxel, float* out_t, uint32_t* visited);
+
+/* The compositing form of rayTrace.wgsl:565-906: labels coloured by the
+ * transfer function (tf_labels ascending, tf_rgba RGBA8; other non-zero
+ * labels get decode.wgsl's hash colour), front-to-back compositing with the
+ * LOD opacity correction, LOD-gradient diffuse shading and shadow rays,
+ * early end at alpha 0.95. brick_labels[i] == NULL: brick i is not resident
+ * (its samples are transparent). out_rgba: w*h RGBA8. */
+int or_render_rgba(uint32_t bricks_per_axis, uint32_t brick_size, const uint8_t* brick_empty,
+                   const uint8_t* brick_lod, const uint64_t* const* brick_labels, const uint64_t* palette0,
+                   const or_camera* cam, uint32_t w, uint32_t h, const uint64_t* tf_labels,
+                   const uint32_t* tf_rgba, uint64_t tf_n, const float light[3], float ambient,
+                   float diffuse, uint32_t* out_rgba);
 
 /* camera.js:103-137 (f64). */
 uint32_t or_brick_lod(double cx, double cy, double cz, double brick_size, double px, double py,

@@ -57,6 +57,10 @@ def lib():
         L.or_render.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_u8p, c_u8p, ctypes.POINTER(c_u64p), c_u64p,
                                 ctypes.POINTER(OrCamera), ctypes.c_uint32, ctypes.c_uint32, c_u64p, c_u64p,
                                 ctypes.POINTER(ctypes.c_float), c_u32p]
+        L.or_render_rgba.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_u8p, c_u8p, ctypes.POINTER(c_u64p), c_u64p,
+                                     ctypes.POINTER(OrCamera), ctypes.c_uint32, ctypes.c_uint32, c_u64p, c_u32p,
+                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_float,
+                                     c_u32p]
         L.or_brick_lod.argtypes = [ctypes.c_double] * 9 + [ctypes.c_uint32]
         L.or_brick_lod.restype = ctypes.c_uint32
         L.or_decode_all_pyramid.argtypes = [ctypes.POINTER(OrDataset), c_u64p, ctypes.c_int, c_u32p]
@@ -160,6 +164,49 @@ def render(bricks_per_axis, brick_size, brick_empty, brick_lod, brick_labels, pa
                     vox.ctypes.data_as(c_u64p), tt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                     vis.ctypes.data_as(c_u32p))
     return lab, vox, tt, vis
+
+
+def _ptrs(brick_labels):
+    ptrs = (c_u64p * len(brick_labels))()
+    keep = []
+    for i, a in enumerate(brick_labels):
+        if a is None:
+            ptrs[i] = c_u64p()
+        else:
+            a = np.ascontiguousarray(a, dtype=np.uint64)
+            keep.append(a)
+            ptrs[i] = a.ctypes.data_as(c_u64p)
+    return ptrs, keep
+
+
+def _camera(cam):
+    c = OrCamera()
+    c.position[:] = cam["position"]
+    c.forward[:] = cam["forward"]
+    c.right[:] = cam["right"]
+    c.up[:] = cam["up"]
+    c.tan_half_fov = cam["tan_half_fov"]
+    c.ortho_half_height = cam["ortho_half_height"]
+    return c
+
+
+def render_rgba(bricks_per_axis, brick_size, brick_empty, brick_lod, brick_labels, palette0, cam, w, h, tf_labels,
+                tf_rgba, light, ambient=0.5, diffuse=0.8):
+    """or_render_rgba: the shaded composite, w*h RGBA8 (uint32)."""
+    ptrs, keep = _ptrs(brick_labels)
+    c = _camera(cam)
+    empty = np.ascontiguousarray(brick_empty, dtype=np.uint8)
+    lods = np.ascontiguousarray(brick_lod, dtype=np.uint8)
+    p0 = np.ascontiguousarray(palette0, dtype=np.uint64)
+    tl = np.ascontiguousarray(tf_labels, dtype=np.uint64)
+    tr = np.ascontiguousarray(tf_rgba, dtype=np.uint32)
+    li = (ctypes.c_float * 3)(*[float(x) for x in light])
+    out = np.zeros(w * h, np.uint32)
+    lib().or_render_rgba(bricks_per_axis, brick_size, empty.ctypes.data_as(c_u8p), lods.ctypes.data_as(c_u8p),
+                         ptrs, p0.ctypes.data_as(c_u64p), ctypes.byref(c), w, h, tl.ctypes.data_as(c_u64p),
+                         tr.ctypes.data_as(c_u32p), tl.size, li, ctypes.c_float(ambient), ctypes.c_float(diffuse),
+                         out.ctypes.data_as(c_u32p))
+    return out
 
 
 def brick_lod(center, brick_size, cam_pos, fov, screen_width, max_lod):
