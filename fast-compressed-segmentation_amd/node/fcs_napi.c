@@ -4,6 +4,7 @@
  * fcs_* entry point and throws a JS Error with fcs_last_error() on failure.
  */
 #include <node_api.h>
+#include <stdbool.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -310,12 +311,12 @@ static napi_value js_orbit(napi_env env, napi_callback_info info) {
   return camera_object(env, &c);
 }
 
-/* render(volume, camera, width, height, brickLods?: Uint8Array)
+/* render(volume, camera, width, height, brickLods?: Uint8Array, shaded?: boolean)
  *   -> { label, voxel (Float64Array, -1 = no hit), tHit, visited, lods,
- *        bricksDecoded, decodeMs, marchMs } (fcs_render) */
+ *        bricksDecoded, decodeMs, marchMs[, rgba: Uint32Array] } (fcs_render) */
 static napi_value js_render(napi_env env, napi_callback_info info) {
-  napi_value argv[5];
-  if (!get_args(env, info, 5, argv)) return NULL;
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return NULL;
   vol_box* vb = unwrap_vol(env, argv[0]);
   if (!vb) return NULL;
   fcs_camera cam;
@@ -345,9 +346,16 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   napi_value ju = new_typed(env, napi_uint8_array, vb->info.num_bricks, 1, &used);
   void* voxd;
   napi_value jx = new_typed(env, napi_float64_array, n, 8, &voxd);
+  bool shaded = false;
+  napi_valuetype st;
+  if (napi_typeof(env, argv[5], &st) == napi_ok && st == napi_boolean) napi_get_value_bool(env, argv[5], &shaded);
+  void* rgbap = NULL;
+  napi_value jr = NULL;
+  if (shaded) jr = new_typed(env, napi_uint32_array, n, 4, &rgbap);
   uint64_t* vox = (uint64_t*)malloc(n * 8);
   fcs_frame f;
   memset(&f, 0, sizeof f);
+  f.rgba = (uint32_t*)rgbap;
   f.label = lab;
   f.voxel = vox;
   f.t_hit = (float*)tt;
@@ -367,6 +375,70 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   set_u32(env, obj, "bricksDecoded", f.bricks_decoded);
   set_f64(env, obj, "decodeMs", f.decode_ms);
   set_f64(env, obj, "marchMs", f.march_ms);
+  if (jr) napi_set_named_property(env, obj, "rgba", jr);
+  return obj;
+}
+
+/* setLighting(volume, lightAngle, ambient, diffuseStrength) (fcs_set_lighting) */
+static napi_value js_set_lighting(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  double a, amb, dif;
+  if (!get_f64(env, argv[1], &a) || !get_f64(env, argv[2], &amb) || !get_f64(env, argv[3], &dif)) {
+    napi_throw_type_error(env, NULL, "setLighting(volume, lightAngle, ambient, diffuseStrength)");
+    return NULL;
+  }
+  fcs_lighting l = {(float)a, (float)amb, (float)dif, 0.0f};
+  int rc = fcs_set_lighting(vb->vol, &l);
+  if (rc < 0) return throw_fcs(env, rc, vb->ctx, "fcs_set_lighting");
+  return NULL;
+}
+
+/* setLabelVisibility(volume, hidden: BigUint64Array, highlight?: bigint)
+ * (fcs_set_label_visibility) */
+static napi_value js_set_label_visibility(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  void* hp = NULL;
+  size_t nh = 0;
+  if (!typed_data(env, argv[1], napi_biguint64_array, &hp, &nh)) {
+    napi_throw_type_error(env, NULL, "hidden labels must be a BigUint64Array");
+    return NULL;
+  }
+  uint64_t hl = 0;
+  const uint64_t* hlp = NULL;
+  napi_valuetype t;
+  if (napi_typeof(env, argv[2], &t) == napi_ok && t == napi_bigint) {
+    bool lossless = false;
+    napi_get_value_bigint_uint64(env, argv[2], &hl, &lossless);
+    hlp = &hl;
+  }
+  int rc = fcs_set_label_visibility(vb->vol, (const uint64_t*)hp, (uint32_t)nh, hlp);
+  if (rc < 0) return throw_fcs(env, rc, vb->ctx, "fcs_set_label_visibility");
+  return NULL;
+}
+
+/* transferFunction(volume) -> { labels: BigUint64Array, rgba: Uint32Array } (fcs_transfer_function) */
+static napi_value js_transfer_function(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  int64_t n = fcs_transfer_function(vb->vol, NULL, NULL, 0);
+  if (n < 0) return throw_fcs(env, (int)n, vb->ctx, "fcs_transfer_function");
+  void *lp, *rp;
+  napi_value jl = new_typed(env, napi_biguint64_array, (size_t)n, 8, &lp);
+  napi_value jr = new_typed(env, napi_uint32_array, (size_t)n, 4, &rp);
+  n = fcs_transfer_function(vb->vol, (uint64_t*)lp, (uint32_t*)rp, (uint64_t)n);
+  if (n < 0) return throw_fcs(env, (int)n, vb->ctx, "fcs_transfer_function");
+  napi_value obj;
+  napi_create_object(env, &obj);
+  napi_set_named_property(env, obj, "labels", jl);
+  napi_set_named_property(env, obj, "rgba", jr);
   return obj;
 }
 
@@ -388,6 +460,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readBrick", NULL, js_read_brick, NULL, NULL, NULL, napi_default, NULL},
       {"orbitCamera", NULL, js_orbit, NULL, NULL, NULL, napi_default, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},
+      {"setLighting", NULL, js_set_lighting, NULL, NULL, NULL, napi_default, NULL},
+      {"setLabelVisibility", NULL, js_set_label_visibility, NULL, NULL, NULL, napi_default, NULL},
+      {"transferFunction", NULL, js_transfer_function, NULL, NULL, NULL, napi_default, NULL},
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -25,6 +25,17 @@ const lods = new Uint8Array(h.numBricks).fill(h.levels);
 const f = fcs.render(ds, cam, 256, 256, lods);
 fs.writeFileSync(path.join(outDir, 'label.bin'), Buffer.from(f.label.buffer));
 fs.writeFileSync(path.join(outDir, 'voxel.bin'), Buffer.from(f.voxel.buffer));
+// shaded frame, then every label hidden (a transparent frame)
+const shaded = fcs.render(ds, cam, 256, 256, lods, true);
+fs.writeFileSync(path.join(outDir, 'rgba.bin'), Buffer.from(shaded.rgba.buffer));
+fcs.updateLightingOptions(ds, { lightAngle: 25 });
+const labels = fcs.getDatasetLabels(ds);
+fcs.applyLabelVisibility(ds, labels);
+fcs.applyLabelHover(ds, labels[0]);
+const hiddenAll = fcs.render(ds, cam, 256, 256, lods, true).rgba.every((x) => x === 0);
+fcs.applyLabelVisibility(ds, []);
+fcs.applyLabelHover(ds, null);
 let threw = false;
 try { fcs.loadDatasetFromArrayBuffer(new Uint8Array(64)); } catch (e) { threw = /Invalid file format/.test(e.message); }
-console.log(JSON.stringify({ header: h, badMagicThrows: threw, marchMs: f.marchMs }));
+console.log(JSON.stringify({ header: h, badMagicThrows: threw, marchMs: f.marchMs, numLabels: labels.length,
+                              hiddenAllTransparent: hiddenAll }));

@@ -17,7 +17,8 @@ ADDON = os.path.join(ROOT, "fast-compressed-segmentation_amd", "node", "fcs.node
 pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the addon missing")
 
 API = ["deviceCount", "createContext", "loadDatasetFromArrayBuffer", "decodeAllRansBricks", "requestBricks",
-       "decodeAll", "readBrick", "orbitCamera", "render"]
+       "decodeAll", "readBrick", "orbitCamera", "render", "updateLightingOptions", "applyLabelVisibility",
+       "applyLabelHover", "getDatasetLabels", "transferFunction"]
 
 
 def test_module_exports_app_api():
@@ -61,3 +62,11 @@ def test_node_path_bit_exact(tmp_path):
     assert np.array_equal(lab.astype(np.uint64), rl)
     rvf = np.where(rv == np.uint64(2 ** 64 - 1), -1.0, rv.astype(np.float64))
     assert np.array_equal(vox, rvf)
+    # shaded frame against the compositing oracle (same tolerance as the Python path)
+    from test_gpu_shading import _compare_rgba, _tf_independent
+    tl, tr = _tf_independent(ds)
+    assert meta["numLabels"] == len(tl) and meta["hiddenAllTransparent"] is True
+    a = 25.0 * np.pi / 180.0
+    light = np.array([np.sin(a), 0.6, -np.cos(a)], np.float32)
+    want = O.render_rgba(4, 32, empty, [5] * 64, labels, p0, cam, 256, 256, tl, tr, light)
+    _compare_rgba(np.fromfile(tmp_path / "rgba.bin", np.uint32), want, "node shaded ortho")
