@@ -85,6 +85,7 @@ struct fcs_volume {
   uint32_t tf_mask = 0;
   float light[3] = {0.0f, 0.0f, 0.0f};
   float ambient = 0.5f, diffuse = 0.8f;
+  uint64_t shade_epoch = 0;     // bumped when the lighting or the table's buffers change
   uint32_t* d_frgba = nullptr;  // frame RGBA (frame_cap pixels)
   // direct-mapped cache: brick i owns voxels [i * stride, (i + 1) * stride)
   bool direct = false;
@@ -357,6 +358,7 @@ void set_lighting(fcs_volume* v, float angle_deg, float ambient, float diffuse) 
   v->light[2] = (float)-std::cos(a);
   v->ambient = ambient;
   v->diffuse = diffuse;
+  ++v->shade_epoch;
 }
 
 // Math.round for a non-negative double (halves go up)
@@ -404,6 +406,7 @@ int ensure_tf(fcs_volume* v) {
     HIP_TRY(c, hipMalloc((void**)&v->d_tf_keys, (size_t)cap * 8));
     HIP_TRY(c, hipMalloc((void**)&v->d_tf_rgba, (size_t)cap * 4));
     v->tf_mask = mask;
+    ++v->shade_epoch;
   }
   // frames that read the table run on the context stream, synchronously
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -717,7 +720,7 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
 
 fcs_ctx* fcs_volume_context(const fcs_volume* v) { return v ? v->ctx : nullptr; }
 
-int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* o) {
+int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* o, bool shading) {
   if (!v || !v->ctx || !o) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -750,6 +753,19 @@ int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* o) {
   o->L = v->L;
   o->B = v->B;
   o->label_bits = v->label_bits;
+  o->tf_keys = nullptr;
+  o->tf_rgba = nullptr;
+  o->tf_mask = 0;
+  if (shading) {
+    if (int rc = ensure_tf(v)) return rc;
+    o->tf_keys = v->d_tf_keys;
+    o->tf_rgba = v->d_tf_rgba;
+    o->tf_mask = v->tf_mask;
+  }
+  for (int k = 0; k < 3; ++k) o->light[k] = v->light[k];
+  o->ambient = v->ambient;
+  o->diffuse = v->diffuse;
+  o->shade_epoch = v->shade_epoch;
   return 0;
 }
 

@@ -65,9 +65,17 @@ struct fcs_volume_desc {
   const void* pal0;     // per brick palette[0]
   const void* cache;    // brick cache pool (may be NULL: nothing resident)
   uint32_t nb, b, L, B, label_bits;
+  // shading (filled when requested): transfer function table and lighting;
+  // shade_epoch changes whenever a value baked into a launch changes
+  const uint64_t* tf_keys;
+  const uint32_t* tf_rgba;
+  uint32_t tf_mask;
+  float light[3], ambient, diffuse;
+  uint64_t shade_epoch;
 };
-// Bring the volume's device residency table up to date (after settling its
-// prefetches; when it changed, the device is synchronised first because
-// frames in flight may read the old table) and describe the volume.
-int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* out);
+// Bring the volume's device residency table (and, with `shading`, its
+// transfer function table) up to date and describe the volume. When the
+// residency changed, the device is synchronised first because frames in
+// flight may read the old table; prefetches are settled first.
+int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* out, bool shading = false);
 fcs_ctx* fcs_volume_context(const fcs_volume* v);

@@ -22,6 +22,7 @@ namespace {
 struct GraphEntry {
   uint64_t serial;
   const void* cache;
+  uint64_t shade_epoch;
   uint32_t w, h;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -43,6 +44,7 @@ struct fcs_view {
   void* d_label = nullptr;    // max_px x 8 bytes (either label width)
   uint64_t* d_voxel = nullptr;
   float* d_t = nullptr;
+  uint32_t* d_rgba = nullptr;  // FCS_VIEW_RGBA: max_px RGBA8
   MarchCam* d_cam = nullptr;
   uint64_t* d_small = nullptr;  // [0] misses (u32), [1] checksum
   // pinned host buffers
@@ -93,6 +95,19 @@ int ensure_bricks(fcs_view* v, uint32_t nb) {
   return 0;
 }
 
+// Shaded frames: the RGBA output, the volume's transfer function and lighting.
+template <typename LT>
+void shade(const fcs_view* v, const fcs_volume_desc& d, MarchArgs<LT>& a) {
+  if (!(v->flags & FCS_VIEW_RGBA)) return;
+  a.out_rgba = v->d_rgba;
+  a.tf_keys = d.tf_keys;
+  a.tf_rgba = d.tf_rgba;
+  a.tf_mask = d.tf_mask;
+  for (int k = 0; k < 3; ++k) a.light[k] = d.light[k];
+  a.ambient = d.ambient;
+  a.diffuse = d.diffuse;
+}
+
 // The frame's device work, in stream order: camera (and fixed LODs) in,
 // per-brick setup, counters cleared by a kernel, march, checksum, outputs out.
 // Identical whether issued directly or recorded into a graph.
@@ -113,6 +128,7 @@ int enqueue_frame(fcs_view* v, const fcs_volume_desc& d, bool fixed_lod, uint32_
     MarchArgs<uint32_t> a{zc, v->d_cam, v->d_meta, v->d_offs, (const uint32_t*)d.pal0, (const uint32_t*)d.cache,
                           (uint32_t*)v->d_label, v->d_voxel, v->d_t, nullptr,
                           reinterpret_cast<uint32_t*>(v->d_small), w, h, (float)w, (float)h, aspect, grid, d.b, d.B};
+    shade(v, d, a);
     VIEW_TRY(v, fcs_launch_march<uint32_t>(a, s));
     if (v->flags & FCS_VIEW_CHECKSUM)
       VIEW_TRY(v, fcs_launch_frame_checksum<uint32_t>((const uint32_t*)v->d_label, v->d_voxel, npx,
@@ -121,20 +137,26 @@ int enqueue_frame(fcs_view* v, const fcs_volume_desc& d, bool fixed_lod, uint32_
     MarchArgs<uint64_t> a{zc, v->d_cam, v->d_meta, v->d_offs, (const uint64_t*)d.pal0, (const uint64_t*)d.cache,
                           (uint64_t*)v->d_label, v->d_voxel, v->d_t, nullptr,
                           reinterpret_cast<uint32_t*>(v->d_small), w, h, (float)w, (float)h, aspect, grid, d.b, d.B};
+    shade(v, d, a);
     VIEW_TRY(v, fcs_launch_march<uint64_t>(a, s));
     if (v->flags & FCS_VIEW_CHECKSUM)
       VIEW_TRY(v, fcs_launch_frame_checksum<uint64_t>((const uint64_t*)v->d_label, v->d_voxel, npx,
                                                       (unsigned long long*)(v->d_small + 1), s));
   }
-  if (v->flags & FCS_VIEW_LABELS)
-    VIEW_TRY(v, hipMemcpyAsync(v->h_label, v->d_label, npx * lb, hipMemcpyDeviceToHost, s));
+  if (v->flags & FCS_VIEW_LABELS) {
+    if (v->flags & FCS_VIEW_RGBA)
+      VIEW_TRY(v, hipMemcpyAsync(v->h_label, v->d_rgba, npx * 4, hipMemcpyDeviceToHost, s));
+    else
+      VIEW_TRY(v, hipMemcpyAsync(v->h_label, v->d_label, npx * lb, hipMemcpyDeviceToHost, s));
+  }
   VIEW_TRY(v, hipMemcpyAsync(v->h_small, v->d_small, 16, hipMemcpyDeviceToHost, s));
   return 0;
 }
 
 GraphEntry* find_graph(fcs_view* v, const fcs_volume_desc& d, uint32_t w, uint32_t h) {
   for (auto& g : v->graphs)
-    if (g.serial == d.serial && g.cache == d.cache && g.w == w && g.h == h) return &g;
+    if (g.serial == d.serial && g.cache == d.cache && g.shade_epoch == d.shade_epoch && g.w == w && g.h == h)
+      return &g;
   return nullptr;
 }
 
@@ -154,7 +176,7 @@ int capture_graph(fcs_view* v, const fcs_volume_desc& d, uint32_t w, uint32_t h,
     (void)hipGraphDestroy(g);
     VIEW_TRY(v, ei);
   }
-  v->graphs.push_back(GraphEntry{d.serial, d.cache, w, h, g, x});
+  v->graphs.push_back(GraphEntry{d.serial, d.cache, d.shade_epoch, w, h, g, x});
   *out = &v->graphs.back();
   return 0;
 }
@@ -169,7 +191,7 @@ extern "C" {
 
 int fcs_view_create(fcs_ctx* ctx, uint32_t max_w, uint32_t max_h, uint32_t flags, fcs_view** out) {
   if (!ctx || !out || max_w == 0 || max_h == 0 || max_w > 16384 || max_h > 16384 ||
-      (flags & ~(FCS_VIEW_GRAPH | FCS_VIEW_LABELS | FCS_VIEW_CHECKSUM)))
+      (flags & ~(FCS_VIEW_GRAPH | FCS_VIEW_LABELS | FCS_VIEW_CHECKSUM | FCS_VIEW_RGBA)))
     return FCS_E_INVALID;
   *out = nullptr;
   FCS_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -190,6 +212,7 @@ int fcs_view_create(fcs_ctx* ctx, uint32_t max_w, uint32_t max_h, uint32_t flags
       bad(hipMalloc((void**)&v->d_cam, sizeof(MarchCam))) || bad(hipMalloc((void**)&v->d_small, 64)) ||
       bad(hipHostMalloc((void**)&v->h_cam, sizeof(MarchCam), hipHostMallocDefault)) ||
       bad(hipHostMalloc((void**)&v->h_small, 64, hipHostMallocDefault)) ||
+      ((flags & FCS_VIEW_RGBA) && bad(hipMalloc((void**)&v->d_rgba, v->max_px * 4))) ||
       ((flags & FCS_VIEW_LABELS) && bad(hipHostMalloc(&v->h_label, v->max_px * 8, hipHostMallocDefault))))
     return fcs_fail(ctx, FCS_E_NOMEM, "view allocation failed");
   *out = v;
@@ -201,7 +224,7 @@ void fcs_view_destroy(fcs_view* v) {
   if (v->ctx) (void)hipSetDevice(v->ctx->device);
   if (v->stream) (void)hipStreamSynchronize(v->stream);
   drop_graphs(v);
-  void* dev[] = {v->d_meta, v->d_offs, v->d_lod, v->d_label, v->d_voxel, v->d_t, v->d_cam, v->d_small};
+  void* dev[] = {v->d_meta, v->d_offs, v->d_lod, v->d_label, v->d_voxel, v->d_t, v->d_cam, v->d_small, v->d_rgba};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {v->h_cam, v->h_label, v->h_small, v->h_lod};
@@ -224,7 +247,7 @@ int fcs_view_render_async(fcs_view* v, fcs_volume* vol, const fcs_camera* cam, c
     if (int rc = fcs_view_wait(v, &r)) return rc;
   }
   fcs_volume_desc d;
-  if (int rc = fcs_volume_publish(vol, &d)) return rc;
+  if (int rc = fcs_volume_publish(vol, &d, (v->flags & FCS_VIEW_RGBA) != 0)) return rc;
   if (int rc = ensure_bricks(v, d.nb)) return rc;
   static_assert(sizeof(MarchCam) == sizeof(fcs_camera), "camera layouts");
   memcpy(v->h_cam, cam, sizeof(MarchCam));
@@ -297,6 +320,16 @@ int fcs_view_read(fcs_view* v, void* label, uint64_t* voxel, float* t_hit, uint8
   }
   VIEW_TRY(v, hipStreamSynchronize(v->stream));
   for (size_t i = 0; i < meta.size(); ++i) brick_lod_used[i] = (uint8_t)(meta[i] & FCS_META_LOD_MASK);
+  return 0;
+}
+
+int fcs_view_read_rgba(fcs_view* v, uint32_t* rgba) {
+  if (!v || !rgba) return FCS_E_INVALID;
+  if (!(v->flags & FCS_VIEW_RGBA)) return vfail(v, FCS_E_INVALID, "not a shaded view");
+  fcs_view_result r;
+  if (int rc = fcs_view_wait(v, &r)) return rc;
+  VIEW_TRY(v, hipMemcpyAsync(rgba, v->d_rgba, (size_t)r.w * r.h * 4, hipMemcpyDeviceToHost, v->stream));
+  VIEW_TRY(v, hipStreamSynchronize(v->stream));
   return 0;
 }
 

@@ -228,7 +228,7 @@ class ServeResultStruct(ctypes.Structure):
     ]
 
 
-VIEW_GRAPH, VIEW_LABELS, VIEW_CHECKSUM = 1, 2, 4
+VIEW_GRAPH, VIEW_LABELS, VIEW_CHECKSUM, VIEW_RGBA = 1, 2, 4, 8
 
 _sig("fcs_view_create", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(c_vp))
 _sig("fcs_view_destroy", None, c_vp)
@@ -239,6 +239,7 @@ _sig("fcs_view_wait", ctypes.c_int, c_vp, ctypes.POINTER(ViewResultStruct))
 _sig("fcs_view_labels", c_vp, c_vp)
 _sig("fcs_view_read", ctypes.c_int, c_vp, c_vp, c_u64p, ctypes.POINTER(ctypes.c_float), c_u8p)
 _sig("fcs_view_graphs", ctypes.c_uint32, c_vp)
+_sig("fcs_view_read_rgba", ctypes.c_int, c_vp, c_u32p)
 _sig("fcs_serve_plan", ctypes.c_int, c_u64p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint64,
      c_u32p, c_u64p)
 _sig("fcs_serve_admit", ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,

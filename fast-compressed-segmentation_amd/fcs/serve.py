@@ -11,6 +11,7 @@ from ._lib import (
     VIEW_CHECKSUM,
     VIEW_GRAPH,
     VIEW_LABELS,
+    VIEW_RGBA,
     CameraStruct,
     ServeRequestStruct,
     ServeResultStruct,
@@ -30,9 +31,11 @@ RESOLUTIONS = ((640, 480), (1280, 720), (1920, 1080))
 class View:
     """A render target with its own HIP stream (fcs_view_create)."""
 
-    def __init__(self, ctx, max_w=1920, max_h=1080, graph=True, labels=False, checksum=True):
+    def __init__(self, ctx, max_w=1920, max_h=1080, graph=True, labels=False, checksum=True, rgba=False):
         self.ctx = ctx
+        self.rgba = rgba
         flags = (VIEW_GRAPH if graph else 0) | (VIEW_LABELS if labels else 0) | (VIEW_CHECKSUM if checksum else 0)
+        flags |= VIEW_RGBA if rgba else 0
         h = c_vp()
         check(lib.fcs_view_create(ctx.h, max_w, max_h, flags, ctypes.byref(h)), ctx.h)
         self.h = h
@@ -66,12 +69,13 @@ class View:
         return {f: getattr(r, f) for f, _ in ViewResultStruct._fields_ if f != "pad"}
 
     def labels(self):
-        """Copy of the pinned label image of the last frame (label views only)."""
+        """Copy of the pinned image of the last frame (label views only): the
+        first-hit labels, or the RGBA8 words of a shaded view."""
         r = self.wait()
         p = lib.fcs_view_labels(self.h)
         if not p:
             return None
-        dt = np.uint64 if r["label_bits"] == 64 else np.uint32
+        dt = np.uint64 if (r["label_bits"] == 64 and not self.rgba) else np.uint32
         n = r["w"] * r["h"]
         return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint8)),
                                      shape=(n * dt().itemsize,)).view(dt).copy()
@@ -89,6 +93,13 @@ class View:
                                 t_hit.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), used.ctypes.data_as(c_u8p)),
               self.ctx.h)
         return Frame(label, voxel, t_hit, None, used, 0, 0.0, r["device_ms"])
+
+    def read_rgba(self):
+        """The last frame's RGBA8 image (shaded views)."""
+        r = self.wait()
+        out = np.zeros(r["w"] * r["h"], np.uint32)
+        check(lib.fcs_view_read_rgba(self.h, out.ctypes.data_as(c_u32p)), self.ctx.h)
+        return out
 
     @property
     def graphs(self):

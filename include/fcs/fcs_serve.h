@@ -35,6 +35,8 @@ typedef struct fcs_view fcs_view;
 #define FCS_VIEW_GRAPH 1u    /* replay each (volume, size) frame as one instantiated hipGraph */
 #define FCS_VIEW_LABELS 2u   /* copy the first-hit label image to the view's pinned host buffer */
 #define FCS_VIEW_CHECKSUM 4u /* compute the fcs_frame_checksum value of every frame on the device */
+#define FCS_VIEW_RGBA 8u     /* shaded frames (the RGBA form of fcs_frame.rgba); with FCS_VIEW_LABELS the
+                                RGBA image is copied to the pinned host buffer instead of the labels */
 
 typedef struct {
   uint64_t checksum;   /* FCS_VIEW_CHECKSUM: equals fcs_frame_checksum of the same frame */
@@ -58,11 +60,14 @@ int fcs_view_query(fcs_view* view);
 /* Wait for the view's last frame and describe it. */
 int fcs_view_wait(fcs_view* view, fcs_view_result* out);
 /* After fcs_view_wait, FCS_VIEW_LABELS views: the pinned host label image of
- * the last frame (w*h labels of label_bits bits), valid until the next frame. */
+ * the last frame (w*h labels of label_bits bits; w*h RGBA8 words for
+ * FCS_VIEW_RGBA views), valid until the next frame. */
 const void* fcs_view_labels(const fcs_view* view);
 /* After fcs_view_wait: copy the last frame's device outputs to host buffers,
  * each nullable (labels of label_bits, voxel index, t, the LOD used per brick). */
 int fcs_view_read(fcs_view* view, void* label, uint64_t* voxel, float* t_hit, uint8_t* brick_lod_used);
+/* After fcs_view_wait, FCS_VIEW_RGBA views: copy the last frame's RGBA8 image. */
+int fcs_view_read_rgba(fcs_view* view, uint32_t* rgba);
 /* hipGraph instantiations the view holds (one per volume and frame size seen). */
 uint32_t fcs_view_graphs(const fcs_view* view);
 

@@ -207,3 +207,29 @@ def test_view_u64_labels(gpu_ctx, manifest):
         _check_equal(view.read(), ref, i)
     view.close()
     vol.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_shaded_view_frames_equal_render(gpu_ctx, vol256, graph):
+    """Shaded views (the RGBA form) equal fcs_render's RGBA output; a lighting
+    change is picked up by graph replays (a new graph is captured)."""
+    vol, _ = vol256
+    view = serve.View(gpu_ctx, 1280, 720, graph=graph, labels=True, checksum=True, rgba=True)
+    cams = _cameras(vol)[:4]
+    try:
+        for angle in (25.0, -40.0):
+            vol.set_lighting(angle, 0.5, 0.8)
+            for i, cam in enumerate(cams):
+                print(f"[gpu] shaded view graph={graph} light {angle} camera {i}", flush=True)
+                ref = vol.render(cam, 640, 480, want=("label", "rgba"))
+                view.render_async(vol, cam, 640, 480)
+                r = view.wait()
+                assert r["misses"] == 0
+                assert np.array_equal(view.labels(), ref.rgba), (angle, i)
+                assert np.array_equal(view.read_rgba(), ref.rgba)
+                assert np.array_equal(view.read().label, ref.label)
+        if graph:
+            assert view.graphs == 2
+    finally:
+        vol.set_lighting()
+        view.close()
