@@ -313,7 +313,9 @@ def bench_config3(args):
         "decode_ms": dec_ms, "rans_ms": rans_ms, "ops_ms": ops_ms,
         "ops_only_gvox_s": voxels / (ops_ms * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": "ops_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("hbm_traffic_2048cube_b32.md", "ops_kernel")
+                     if gen.size == 2048 else None},
         "rans_gbs": rans_bytes / (rans_ms * 1e-3) / 1e9,
         "rans_autotune": {"mapping": tune["rans_mapping"], "lane_ms": tune["tune_lane_ms"],
                           "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],
