@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "fast-compressed-segmentation_amd"))
 
 HBM_PEAK_GBS = 8000.0
+RANS_MAPPINGS = {1: "lane per brick", 2: "wave per brick (scalar table)", 3: "wave per brick (ballot)",
+                 4: "wave per brick (LDS table)", 5: "wave per brick, scalar byte window (table)",
+                 6: "wave per brick, scalar byte window (ballot)"}
 
 
 def log(*a):
@@ -182,11 +185,11 @@ def main():
                      "frac": ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic("hbm_traffic_512cube_b64.md", dom)},
         "kernels": kernels,
-        "rans_autotune": {"mapping": {1: "lane per brick", 2: "wave per brick (scalar table)",
-                                      3: "wave per brick (ballot)", 4: "wave per brick (LDS table)"
-                                      }.get(tune["rans_mapping"]),
+        "rans_autotune": {"mapping": RANS_MAPPINGS.get(tune["rans_mapping"]),
                           "lane_ms": tune["tune_lane_ms"], "wave_table_ms": tune["tune_wave_ms"],
-                          "wave_ballot_ms": tune["tune_ballot_ms"], "wave_lds_ms": tune["tune_wave_lds_ms"]},
+                          "wave_ballot_ms": tune["tune_ballot_ms"], "wave_lds_ms": tune["tune_wave_lds_ms"],
+                          "wave_window_table_ms": tune["tune_win_ms"],
+                          "wave_window_ballot_ms": tune["tune_win_ballot_ms"]},
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -317,9 +320,10 @@ def bench_config3(args):
                      "traffic": measured_traffic("hbm_traffic_2048cube_b32.md", "ops_kernel")
                      if gen.size == 2048 else None},
         "rans_gbs": rans_bytes / (rans_ms * 1e-3) / 1e9,
-        "rans_autotune": {"mapping": tune["rans_mapping"], "lane_ms": tune["tune_lane_ms"],
+        "rans_autotune": {"mapping": RANS_MAPPINGS.get(tune["rans_mapping"]), "lane_ms": tune["tune_lane_ms"],
                           "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],
-                          "wave_lds_ms": tune["tune_wave_lds_ms"]},
+                          "wave_lds_ms": tune["tune_wave_lds_ms"], "wave_window_table_ms": tune["tune_win_ms"],
+                          "wave_window_ballot_ms": tune["tune_win_ballot_ms"]},
         "ops_autotune": {"split": {1: "3 workgroups/CU", 2: "4 workgroups/CU"}.get(tops["ops_split"]),
                          "3wg_ms": tops["tune_ops_3wg_ms"], "4wg_ms": tops["tune_ops_4wg_ms"]},
         "batched": {"requests": int(nreq), "lods": "uniform 0..%d" % L, "ms": batch_ms,

@@ -554,7 +554,8 @@ int fcs_sync(fcs_ctx* c) {
 int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
   if (!c) return FCS_E_INVALID;
   if (option == FCS_OPT_RANS_MAPPING) {
-    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_LDS) return fail(c, FCS_E_INVALID, "bad mapping");
+    if (value < FCS_RANS_MAPPING_AUTO || value > FCS_RANS_MAPPING_WAVE_WIN_BALLOT)
+      return fail(c, FCS_E_INVALID, "bad mapping");
     c->rans_mapping = (int)value;
     return 0;
   }
@@ -845,17 +846,21 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
       // after a one-brick launch of each so code loading is not timed
       RansArgs one = a;
       one.nlist = 1;
-      const int modes[4] = {FCS_RANS_WAVE, FCS_RANS_WAVE_BALLOT, FCS_RANS_WAVE_LDS, FCS_RANS_LANE};
+      constexpr int kModes = 6;
+      const int modes[kModes] = {FCS_RANS_WAVE, FCS_RANS_WAVE_BALLOT, FCS_RANS_WAVE_LDS, FCS_RANS_LANE,
+                                 FCS_RANS_WIN, FCS_RANS_WIN_BALLOT};
       for (int md : modes) HIP_TRY(c, fcs_launch_rans(one, md, c->stream));
-      float t[4] = {0, 0, 0, 0};
-      for (int k = 0; k < 4; ++k)
+      float t[kModes] = {0, 0, 0, 0, 0, 0};
+      for (int k = 0; k < kModes; ++k)
         if (int rc = timed(modes[k], &t[k])) return rc;
       c->timing.tune_wave_ms = t[0];
       c->timing.tune_ballot_ms = t[1];
       c->timing.tune_wave_lds_ms = t[2];
       c->timing.tune_lane_ms = t[3];
+      c->timing.tune_win_ms = t[4];
+      c->timing.tune_win_ballot_ms = t[5];
       int best = 0;
-      for (int k = 1; k < 4; ++k)
+      for (int k = 1; k < kModes; ++k)
         if (t[k] < t[best]) best = k;
       v->rans_tuned = modes[best];
     }
@@ -1047,6 +1052,31 @@ int fcs_read_cache(fcs_volume* v, uint64_t off, uint64_t count, void* host) {
                             hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int64_t fcs_read_nibbles(fcs_volume* v, uint32_t brick, uint8_t* out, uint64_t cap) {
+  if (!v || !v->ctx || brick >= v->nb) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  if (!v->rans_ready) return fail(c, FCS_E_INVALID, "fcs_decode_rans has not run");
+  const uint32_t nsym = v->hb[brick].nsym;
+  const uint64_t nbytes = ((uint64_t)nsym + 1) / 2;
+  if (!out) return (int64_t)nbytes;
+  if (cap < nbytes) return fail(c, FCS_E_INVALID, "output buffer too small");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
+  std::vector<uint32_t> w(((uint64_t)nsym + 7) / 8);
+  if (!w.empty()) {
+    HIP_TRY(c, hipMemcpyAsync(w.data(), v->d_nib + v->hb[brick].nib_off, w.size() * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  for (uint64_t j = 0; j < nbytes; ++j) {
+    const uint64_t k0 = 2 * j, k1 = 2 * j + 1;
+    const uint32_t hi = (w[k0 >> 3] >> (4 * (k0 & 7))) & 0xFu;
+    const uint32_t lo = k1 < nsym ? (w[k1 >> 3] >> (4 * (k1 & 7))) & 0xFu : 0u;
+    out[j] = (uint8_t)((hi << 4) | lo);
+  }
+  return (int64_t)nbytes;
 }
 
 int fcs_read_brick(fcs_volume* v, uint32_t brick, uint32_t lod, void* host) {

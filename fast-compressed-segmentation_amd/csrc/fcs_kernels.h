@@ -11,6 +11,8 @@
 #define FCS_RANS_WAVE 2  // one wave per brick, scalar-cache decode table
 #define FCS_RANS_WAVE_BALLOT 3  // one wave per brick, ballot over a per-lane CDF
 #define FCS_RANS_WAVE_LDS 4  // one wave per brick, decode table in LDS
+#define FCS_RANS_WIN 5  // one wave per brick, scalar byte window, scalar-cache decode table
+#define FCS_RANS_WIN_BALLOT 6  // one wave per brick, scalar byte window, ballot over a per-lane CDF
 
 struct RansArgs {
   const DevBrick* bricks;

@@ -171,6 +171,149 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
   if (lane == 0) a.res[b] = RansResult{status, 0u};
 }
 
+// ------------------------------------------------- wave per brick, window
+// Same mapping, with the byte source in scalar registers: a 64-bit window of
+// upcoming stream bytes (next byte in bits 63..56) refilled 4 bytes at a time
+// from a 64-word block held one word per lane (v_readlane), the next block
+// loading while the current one is consumed; no LDS. A group of 8 symbols
+// whose start has the state >= 2^23 and >= 16 bytes left needs at most 2
+// bytes per symbol: its symbol step is straight scalar code and the
+// renormalisation runs out of line. Other groups (stream end, malformed
+// input) take the byte-by-byte loop of the reference.
+// SYM: 0 = decode table through the scalar cache, 1 = ballot over a per-lane CDF
+template <int SYM>
+__global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t b = a.list ? a.list[blockIdx.x] : blockIdx.x;
+  const DevBrick B = a.bricks[b];
+  const uint32_t E = B.E, nsym = B.nsym;
+  if (E < 4 || nsym == 0) {
+    if (lane == 0) a.res[b] = RansResult{FCS_ST_RANS_STATE | FCS_ST_STREAM_OVERRUN, 0u};
+    return;
+  }
+  uint32_t* nibw = a.nib + B.nib_off;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(a.stream + B.str_off);
+  const uint32_t my_next = lane < 15 ? a.cum[lane + 1] : 0xFFFFFFFFu;
+  const uint32_t my_f = lane < 16 ? a.freq[lane] : 0u;
+  const uint32_t my_c = lane < 16 ? a.cum[lane] : 0u;
+  typedef __attribute__((address_space(4))) const uint32_t ctab_t;
+  ctab_t* tab = (ctab_t*)a.table;
+
+  // lane i of a block holds word (base - i); words before the stream read 0
+  int32_t base = (int32_t)((E + 3) >> 2) - 1;
+  auto ldw = [&](int32_t top) -> uint32_t {
+    const int32_t w = top - (int32_t)lane;
+    return w >= 0 ? words[w] : 0u;
+  };
+  uint32_t blk = ldw(base), nblk = ldw(base - 64);
+  uint32_t widx = 0;
+  auto next_word = [&]() -> uint32_t {
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)blk, (int)widx);
+    if (++widx == 64u) {
+      blk = nblk;
+      base -= 64;
+      nblk = ldw(base - 64);
+      widx = 0;
+    }
+    return w;
+  };
+  uint64_t win;
+  uint32_t avail;
+  {
+    const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of the last word beyond E
+    win = (uint64_t)(next_word() << (8u * skip)) << 32;
+    avail = 32u - 8u * skip;
+  }
+  // avail <= 32 before every refill
+  auto refill = [&]() {
+    win |= (uint64_t)next_word() << (32u - avail);
+    avail += 32u;
+  };
+  refill();
+  uint32_t pos = E - 4;  // bytes left after the initial state
+  uint32_t state = (uint32_t)(win >> 32);
+  win <<= 32;
+  avail -= 32u;
+
+  auto lookup = [&](uint32_t x, uint32_t& f, uint32_t& r) -> uint32_t {
+    if (SYM == 0) {
+      const uint32_t e = tab[x];
+      f = (e >> 12) & 0x1FFFu;
+      r = e & 0xFFFu;
+      return e >> 25;
+    } else {
+      const uint32_t sym = (uint32_t)__popcll(__ballot(x >= my_next));
+      f = (uint32_t)__builtin_amdgcn_readlane((int)my_f, (int)sym);
+      r = x - (uint32_t)__builtin_amdgcn_readlane((int)my_c, (int)sym);
+      return sym;
+    }
+  };
+  auto fast_one = [&]() -> uint32_t {
+    uint32_t f, r;
+    const uint32_t sym = lookup(state & (FCS_RANS_M - 1), f, r);
+    uint32_t s1 = f * (state >> FCS_RANS_LOGM) + r;
+    if (__builtin_expect(s1 < FCS_RANS_L, 0)) {
+      if (avail < 16u) refill();
+      const uint32_t two = (uint32_t)(win >> 48);
+      if (s1 < (1u << 15)) {
+        s1 = (s1 << 16) | two;
+        win <<= 16;
+        avail -= 16u;
+        pos -= 2;
+      } else {
+        s1 = (s1 << 8) | (two >> 8);
+        win <<= 8;
+        avail -= 8u;
+        pos -= 1;
+      }
+    }
+    state = s1;
+    return sym;
+  };
+  auto exact_one = [&]() -> uint32_t {
+    uint32_t f, r;
+    const uint32_t sym = lookup(state & (FCS_RANS_M - 1), f, r);
+    state = f * (state >> FCS_RANS_LOGM) + r;
+    while (state < FCS_RANS_L && pos > 0) {
+      if (avail < 8u) refill();
+      state = (state << 8) | (uint32_t)(win >> 56);
+      win <<= 8;
+      avail -= 8u;
+      --pos;
+    }
+    return sym;
+  };
+
+  uint32_t outv = 0;
+  const uint32_t nfull = nsym >> 3;
+  for (uint32_t wd = 0; wd < nfull; ++wd) {
+    uint32_t word = 0;
+    if (pos >= 16u && state >= FCS_RANS_L) {
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) word |= fast_one() << (4u * k);
+    } else {
+      for (uint32_t k = 0; k < 8; ++k) word |= exact_one() << (4u * k);
+    }
+    outv = lane == (wd & 63u) ? word : outv;
+    if ((wd & 63u) == 63u) nibw[(wd >> 6) * 64u + lane] = outv;
+  }
+  if (nsym & 7u) {
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < (nsym & 7u); ++k) word |= exact_one() << (4u * k);
+    outv = lane == (nfull & 63u) ? word : outv;
+  }
+  const uint32_t total_words = (nsym + 7) >> 3;
+  const uint32_t stored = (nsym >> 9) * 64u;
+  if (lane < total_words - stored) nibw[stored + lane] = outv;
+
+  uint32_t status = 0;
+  if (nsym & 1u) {
+    if (exact_one() != 0) status |= FCS_ST_RANS_STATE;
+  }
+  if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  if (lane == 0) a.res[b] = RansResult{status, 0u};
+}
+
 // ---------------------------------------------------------- lane per brick
 // Backward byte reader: 128 bytes (8 blocks of 16) are fetched per refill and
 // consumed through register shifts, so a lane waits for memory once per 128
@@ -325,6 +468,10 @@ hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s) {
     hipLaunchKernelGGL(rans_wave_kernel<1>, dim3(a.nlist), dim3(64), 0, s, a);
   } else if (mode == FCS_RANS_WAVE_LDS) {
     hipLaunchKernelGGL(rans_wave_kernel<2>, dim3(a.nlist), dim3(64), 0, s, a);
+  } else if (mode == FCS_RANS_WIN) {
+    hipLaunchKernelGGL(rans_win_kernel<0>, dim3(a.nlist), dim3(64), 0, s, a);
+  } else if (mode == FCS_RANS_WIN_BALLOT) {
+    hipLaunchKernelGGL(rans_win_kernel<1>, dim3(a.nlist), dim3(64), 0, s, a);
   } else {
     hipLaunchKernelGGL(rans_lane_kernel, dim3((a.nlist + 255) / 256), dim3(256), 0, s, a);
   }

@@ -112,11 +112,14 @@ class TimingStruct(ctypes.Structure):
         ("ops_split", ctypes.c_uint32),
         ("tune_ops_3wg_ms", ctypes.c_float),
         ("tune_ops_4wg_ms", ctypes.c_float),
+        ("tune_win_ms", ctypes.c_float),
+        ("tune_win_ballot_ms", ctypes.c_float),
         ("pad", ctypes.c_uint32),
     ]
 
 
-RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT, RANS_WAVE_LDS = 0, 1, 2, 3, 4
+RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT, RANS_WAVE_LDS, RANS_WAVE_WIN, RANS_WAVE_WIN_BALLOT = \
+    0, 1, 2, 3, 4, 5, 6
 OPT_RANS_MAPPING = 1
 OPT_OPS_SPLIT = 2
 OPS_SPLIT_AUTO, OPS_SPLIT_3WG, OPS_SPLIT_4WG = 0, 1, 2
@@ -166,6 +169,7 @@ _sig("fcs_request_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32, c
 _sig("fcs_decode_all", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_int)
 _sig("fcs_read_brick", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, c_vp)
 _sig("fcs_read_cache", ctypes.c_int, c_vp, ctypes.c_uint64, ctypes.c_uint64, c_vp)
+_sig("fcs_read_nibbles", ctypes.c_int64, c_vp, ctypes.c_uint32, c_u8p, ctypes.c_uint64)
 _sig("fcs_brick_lods", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), ctypes.c_uint32, ctypes.c_uint32, c_u8p)
 _sig("fcs_render", ctypes.c_int, c_vp, ctypes.POINTER(CameraStruct), c_u8p, ctypes.c_uint32, ctypes.c_uint32,
      ctypes.POINTER(FrameStruct))

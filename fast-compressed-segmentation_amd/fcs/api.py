@@ -95,7 +95,8 @@ class Context:
         return {f: getattr(t, f) for f, _ in TimingStruct._fields_}
 
     def set_rans_mapping(self, mapping):
-        """0 = autotuned, 1 = one lane per brick, 2/3/4 = one wave per brick (scalar-cache table / ballot / LDS table)."""
+        """0 = autotuned, 1 = one lane per brick, 2/3/4 = one wave per brick (scalar-cache table / ballot / LDS
+        table), 5/6 = one wave per brick with the byte window in scalar registers (table / ballot)."""
         check(lib.fcs_set_option(self.h, 1, int(mapping)), self.h)
 
     def set_ops_split(self, split):
@@ -180,6 +181,14 @@ class Volume:
     def read_brick(self, brick, lod):
         out = np.zeros(1 << (3 * lod), self.label_dtype)
         check(lib.fcs_read_brick(self.h, int(brick), int(lod), _ptr(out, c_vp)), self.ctx.h)
+        return out
+
+    def read_nibbles(self, brick):
+        """Nibble stream of a brick after decode_rans, two nibbles per byte,
+        first in the high half (decodeAllRansBricks, web/scripts/decode.js:17-26)."""
+        n = check(lib.fcs_read_nibbles(self.h, int(brick), None, 0), self.ctx.h)
+        out = np.zeros(n, np.uint8)
+        check(lib.fcs_read_nibbles(self.h, int(brick), _ptr(out, c_u8p), n), self.ctx.h)
         return out
 
     def brick_lods(self, cam, w, h):

@@ -138,6 +138,8 @@ typedef struct {
   uint32_t ops_split;     /* FCS_OPS_SPLIT_* the op decoder used last (bricks up to 32^3) */
   float tune_ops_3wg_ms;  /* autotuner: op decode time of the first large batch, 3 workgroups/CU */
   float tune_ops_4wg_ms;  /* autotuner: the same batch, 4 workgroups/CU */
+  float tune_win_ms;      /* autotuner: whole-volume rANS time, one wave per brick, scalar byte window (table) */
+  float tune_win_ballot_ms; /* autotuner: same, ballot over a per-lane CDF */
   uint32_t pad;
 } fcs_timing;
 
@@ -148,6 +150,8 @@ typedef struct {
 #define FCS_RANS_MAPPING_WAVE 2 /* one wave per brick (few, long bricks), scalar decode table */
 #define FCS_RANS_MAPPING_WAVE_BALLOT 3 /* one wave per brick, ballot over a per-lane CDF */
 #define FCS_RANS_MAPPING_WAVE_LDS 4 /* one wave per brick, decode table in LDS */
+#define FCS_RANS_MAPPING_WAVE_WIN 5 /* one wave per brick, byte window in scalar registers, scalar decode table */
+#define FCS_RANS_MAPPING_WAVE_WIN_BALLOT 6 /* one wave per brick, byte window in scalar registers, ballot */
 #define FCS_OPT_OPS_SPLIT 2 /* FCS_OPS_SPLIT_*; default AUTO */
 #define FCS_OPS_SPLIT_AUTO 0 /* time both splits on a volume's first batch of >= 2048 requests, keep the faster */
 #define FCS_OPS_SPLIT_3WG 1 /* op decoder LDS split for three workgroups per CU (bricks up to 32^3) */
@@ -195,6 +199,11 @@ int fcs_read_brick(fcs_volume* vol, uint32_t brick, uint32_t lod, void* host_lab
 
 /* Copy the whole brick cache pool region [offset, offset+count) to the host. */
 int fcs_read_cache(fcs_volume* vol, uint64_t offset, uint64_t count, void* host_labels);
+/* decodeAllRansBricks's per-brick result (web/scripts/decode.js:17-26): the
+ * nibble stream of `brick` after fcs_decode_rans, nSymbols nibbles packed two
+ * per byte with the first nibble in the high half ((nSymbols+1)/2 bytes; a
+ * trailing odd nibble leaves the low half 0). Returns the byte count. */
+int64_t fcs_read_nibbles(fcs_volume* vol, uint32_t brick, uint8_t* out, uint64_t cap);
 
 /* Per-brick LOD by camera.js:103-137 (f64) for a w x h viewport. */
 int fcs_brick_lods(const fcs_volume* vol, const fcs_camera* cam, uint32_t w, uint32_t h,

@@ -131,10 +131,11 @@ def test_config2_full_decode_bit_exact(gpu_ctx, cfg2):
         assert np.array_equal(cache[i, O.pyramid_offset(6):].astype(np.uint64), brick_from_volume(src, i, 64))
 
 
-@pytest.mark.parametrize("mapping", [1, 2, 3, 4])
-@pytest.mark.parametrize("name", ["cfg1_b32", "microns8_b64", "checker_b16", "const_mix_b8"])
+@pytest.mark.parametrize("mapping", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("name", ["cfg1_b32", "microns8_b64", "checker_b16", "const_mix_b8", "rand_b2", "lookback16_b8"])
 def test_both_rans_mappings_bit_exact(gpu_ctx, manifest, name, mapping):
-    """Lane-per-brick (1) and wave-per-brick (2: scalar table, 3: ballot, 4: LDS table) rANS kernels, forced."""
+    """Lane-per-brick (1) and wave-per-brick (2: scalar table, 3: ballot, 4: LDS table, 5/6: scalar byte
+    window with table / ballot) rANS kernels, forced."""
     print(f"[gpu] rANS mapping {mapping} on {name}", flush=True)
     gpu_ctx.set_rans_mapping(mapping)
     try:
@@ -154,7 +155,7 @@ def test_both_rans_mappings_bit_exact(gpu_ctx, manifest, name, mapping):
 def test_config2_both_mappings(gpu_ctx, cfg2):
     _, data = cfg2
     outs = []
-    for mapping in (1, 2, 3, 4):
+    for mapping in (1, 2, 3, 4, 5, 6):
         gpu_ctx.set_rans_mapping(mapping)
         vol = gpu_ctx.load(data)
         assert np.all(vol.decode_rans() == 0)
@@ -213,3 +214,36 @@ def test_ops_split_autotune_512cube_b32(gpu_ctx, cfg2):
     cache = outs[0].reshape(4096, 32768)
     for i in (0, 1000, 4095):
         assert np.array_equal(cache[i].astype(np.uint64), brick_from_volume(src, i, 32))
+
+
+@pytest.mark.parametrize("mapping", [1, 2, 3, 4, 5, 6])
+def test_rans_nibbles_equal_oracle_with_corrupt_bricks(gpu_ctx, manifest, mapping):
+    """Every rANS mapping's nibble streams (fcs_read_nibbles) equal the
+    oracle's decodeRansStream output byte for byte, with per-brick statuses,
+    also for bricks whose stream was corrupted (flipped bytes near the start,
+    middle and end) -- the byte-exact end-of-stream paths."""
+    print(f"[gpu] nibbles vs oracle, mapping {mapping}", flush=True)
+    for name, b in (("cfg1_b32", 32), ("microns8_b64", 64)):
+        data = bytearray(golden_bytes(name))
+        ds0 = O.Dataset(bytes(data), b)
+        nb = ds0.num_bricks
+        bad = [1, nb // 2, nb - 1]
+        for k, i in enumerate(bad):
+            br = ds0.brick(i)
+            if br.E > 8:
+                data[br.str_off + [1, br.E // 2, br.E - 6][k]] ^= 0x5A
+        ds = O.Dataset(bytes(data), b)
+        gpu_ctx.set_rans_mapping(mapping)
+        try:
+            vol = gpu_ctx.load(bytes(data), b)
+            st = vol.decode_rans(raise_on_error=False)
+            assert gpu_ctx.timing()["rans_mapping"] == mapping
+            for i in range(nb):
+                want, wst = ds.rans(i)
+                got = vol.read_nibbles(i)
+                assert np.array_equal(got, want[:got.size]) and got.size == (ds.brick(i).nsym + 1) // 2, (name, i)
+                assert (st[i] != 0) == (wst != 0), (name, i, st[i], wst)
+            vol.close()
+        finally:
+            gpu_ctx.set_rans_mapping(0)
+
