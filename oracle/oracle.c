@@ -135,9 +135,10 @@ static uint32_t rans_step(uint32_t* state, const uint8_t* in, size_t* pos, const
 }
 
 size_t or_rans_decode(const or_dataset* d, uint32_t brick, uint8_t* out, uint32_t* status) {
+  *status = 0;
+  if (brick >= d->num_bricks) { *status = ST_RANS; return 0; }
   const or_brick* b = &d->bricks[brick];
   const uint8_t* in = d->src + b->str_off;
-  *status = 0;
   if (b->E < 4) { *status |= ST_RANS; return 0; }
   size_t pos = b->E;
   const uint32_t b0 = in[--pos], b1 = in[--pos], b2 = in[--pos], b3 = in[--pos];
@@ -271,6 +272,7 @@ done:
 
 int or_decode_dataset_brick(const or_dataset* d, uint32_t brick, uint32_t t, uint64_t* out,
                             uint32_t* status) {
+  if (brick >= d->num_bricks) { *status = ST_OVERRUN; return -1; }
   const or_brick* b = &d->bricks[brick];
   uint8_t* nib = (uint8_t*)malloc((b->nsym + 2) / 2 + 1);
   uint32_t st1 = 0, st2 = 0;
@@ -611,6 +613,9 @@ static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uin
         const uint32_t lod = sc->lod[bi];
         const int lodSize = 1 << lod;
         const int stride = (int)b / lodSize;
+        /* a brick without labels at its LOD is not resident: the ray passes
+         * through it (the device counts a miss there) */
+        const int resident = lod == 0 || sc->labels[bi] != NULL;
         for (int a = 0; a < 3; ++a) vox[a] -= bmin[a];
         if (stride > 1 && OR_MUTANT != 4) {
           for (int a = 0; a < 3; ++a) {
@@ -639,7 +644,7 @@ static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uin
         }
         float tcur = tEnterBrick;
         const float lodScale = (float)lodSize / (float)b;
-        for (uint32_t s = 0; s < maxs; ++s) {
+        for (uint32_t s = 0; s < (resident ? maxs : 0u); ++s) {
           const int inb = vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < (int)b && vox[1] < (int)b &&
                           vox[2] < (int)b;
           if (inb) {

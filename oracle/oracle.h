@@ -73,7 +73,8 @@ typedef struct {
 
 /* rayTrace.wgsl traversal up to the first hit (label != 0).
  * brick_labels[i] points at brick i's labels at LOD brick_lod[i] (Morton
- * order; unused for empty bricks or LOD 0, where palette0[i] is used). */
+ * order; unused for empty bricks or LOD 0, where palette0[i] is used);
+ * NULL for a non-empty brick of LOD > 0 = not resident: rays pass through. */
 int or_render(uint32_t bricks_per_axis, uint32_t brick_size, const uint8_t* brick_empty,
               const uint8_t* brick_lod, const uint64_t* const* brick_labels,
               const uint64_t* palette0, const or_camera* cam, uint32_t w, uint32_t h,

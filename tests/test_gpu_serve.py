@@ -10,6 +10,7 @@ import pytest
 import fcs
 from conftest import golden_bytes
 from fcs import paths, serve
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -127,6 +128,26 @@ def test_view_misses_and_fixed_lods(gpu_ctx, vol256):
     # the LOD rule asks for LOD 5 near the camera: those bricks are not resident
     view.render_async(vol, cam, 640, 480)
     assert view.wait()["misses"] > 0
+    # rays pass through non-resident bricks: the frame equals the oracle's
+    # with those bricks absent
+    vol.cache_clear()
+    keep = np.arange(0, nb, 3, dtype=np.uint32)
+    vol.request_bricks(keep, np.full(keep.size, 3, np.uint8))
+    view.render_async(vol, cam, 640, 480, brick_lod=fixed)
+    assert view.wait()["misses"] > 0
+    fr = view.read()
+    ds = O.Dataset(data)
+    pyr, _ = ds.decode_all_pyramid()
+    B = round(nb ** (1 / 3))
+    labels, empty, p0 = [], [], []
+    for i in range(nb):
+        pal = ds.palette(i)
+        empty.append(int(np.all(pal == 0)))
+        p0.append(pal[0])
+        labels.append(pyr[i][O.pyramid_offset(3):O.pyramid_offset(4)] if i % 3 == 0 else None)
+    lab, vox, tt, _ = O.render(B, 32, empty, fixed, labels, p0, cam.as_dict(), 640, 480)
+    assert np.array_equal(fr.label.astype(np.uint64), lab) and np.array_equal(fr.voxel, vox)
+    assert np.array_equal(fr.t_hit.view(np.uint32), tt.view(np.uint32))
     view.close()
     vol.close()
 
