@@ -59,6 +59,7 @@ using CfgLarge = OpsCfg<1024, 4096, 0, 32768, 32768, false, 1024, 0>;
 constexpr uint32_t SCR_WORDS = 256;
 constexpr uint32_t SCR_WAVESUM = 0;   // NW words: active-node counts per wave
 constexpr uint32_t SCR_REC = 16;      // 2 x 48 words: parse records per wave
+constexpr uint32_t SCR_HAND = 112;    // 5 words: level state handed from the first wave to the workgroup
 constexpr uint32_t SCR_STATUS = 120;
 constexpr uint32_t SCR_NS = 122;      // nibble where the next level starts
 constexpr uint32_t SCR_IP = 123;      // palette cursor at the next level's start
@@ -101,6 +102,14 @@ struct Lds {
 
 enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2 };
 
+// The same configuration run by one wave (the first of the workgroup): its
+// LDS layout is the workgroup's, its thread count 64 and its syncs wave-level.
+template <class C>
+struct OneWave : C {
+  static constexpr uint32_t NT = 64;
+  static constexpr uint32_t NW = 1;
+};
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Workgroup barrier that orders LDS only: global loads stay in flight across
@@ -111,6 +120,19 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The sync of a (sub-)configuration: a workgroup barrier, or for one wave an
+// ordering point only (a wave's LDS accesses complete in program order).
+template <class C>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (C::NW == 1) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  } else {
+    lds_barrier();
+  }
 }
 
 // inclusive wave scan on the VALU data-parallel-primitive lanes: shifts by
@@ -145,18 +167,22 @@ __device__ __forceinline__ uint4 splat16(LT v) {
   }
 }
 
+// 16-byte store as one clang vector: a uint4 struct store built from
+// separate values is emitted as four dword stores
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename LT>
 __device__ __forceinline__ void store8(LT* p, const LT (&v)[8], bool al) {
   if (al) {
-    uint4* q = reinterpret_cast<uint4*>(p);
+    u32x4* q = reinterpret_cast<u32x4*>(__builtin_assume_aligned(p, 16));
     if constexpr (sizeof(LT) == 4) {
-      q[0] = make_uint4(v[0], v[1], v[2], v[3]);
-      q[1] = make_uint4(v[4], v[5], v[6], v[7]);
+      q[0] = u32x4{v[0], v[1], v[2], v[3]};
+      q[1] = u32x4{v[4], v[5], v[6], v[7]};
     } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        q[i] = make_uint4((uint32_t)v[2 * i], (uint32_t)(v[2 * i] >> 32), (uint32_t)v[2 * i + 1],
-                          (uint32_t)(v[2 * i + 1] >> 32));
+        q[i] = u32x4{(uint32_t)v[2 * i], (uint32_t)(v[2 * i] >> 32), (uint32_t)v[2 * i + 1],
+                     (uint32_t)(v[2 * i + 1] >> 32)};
     }
   } else {
 #pragma unroll
@@ -348,7 +374,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
       R[3 * wave + 1] = i0;
       R[3 * wave + 2] = i1;
     }
-    lds_barrier();
+    group_sync<C>();
     uint32_t X = carry, base = 0, myX = 0, mybase = 0;
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) {
@@ -370,7 +396,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
       uint32_t end_g = 0, end_ip = 0;
       uint16_t* dum16 = reinterpret_cast<uint16_t*>(S.scr + SCR_DUM16) + lane;
       uint32_t* dum32 = S.scr + SCR_DUM32 + lane;
-#pragma unroll
+#pragma unroll 2
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t g = g0 + j;
         const uint32_t v = (word >> (4 * j)) & 0xFu;
@@ -444,6 +470,247 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
   return op_base;
 }
 
+
+// Per-request state carried from level to level.
+template <typename LT>
+struct LevelVars {
+  uint16_t* vcur;    // values of the current level (u16 palette indices)
+  uint32_t* scur;    // stop bits of the current level
+  uint32_t* snext;   // stop bits of the next level
+  uint32_t A;        // active (not stopped) nodes of the current level
+  uint32_t ns;       // nibble where the current level's ops start
+  uint32_t ip0;      // palette cursor at that nibble
+  uint32_t tile;     // parse tiles so far (record double-buffering)
+};
+
+// Per-request constants of the level loop.
+template <typename LT>
+struct LevelConst {
+  const uint32_t* nibw;
+  uint32_t nsym, w_end, t, L, P;
+  const LT* pal;   // palette in global memory
+  const LT* lp;    // palette as read by the output (LDS when staged)
+  LT* o;           // request output
+  bool pyr, staged;
+};
+
+// Every level from lv+1 to the target (or only the target) copies level lv.
+template <typename LT, class C>
+__device__ __forceinline__ void expand_rest(const Lds<LT>& S, const LevelConst<LT>& c, const uint16_t* vl,
+                                            uint32_t lv) {
+  for (uint32_t lt = c.pyr ? lv + 1 : c.t; lt <= c.t; ++lt) {
+    LT* ol = c.o + (c.pyr ? fcs_pyramid_level_offset(lt) : 0ull);
+    if (c.staged) expand_level<LT, C::NT>(ol, lt, lv, vl, S.spal, c.P);
+    else expand_level<LT, C::NT>(ol, lt, lv, vl, c.pal, c.P);
+  }
+}
+
+// One level step (level l -> l+1) run by the threads of CS: the workgroup
+// (CS = C) or its first wave (CS = OneWave<C>, non-final levels only, the
+// end-of-brick expansion left to the caller). Three phases: stopped
+// children and the parse/resolve of the level's ops (P1), reference chasing
+// (P2), the next level's active list or the labels (P3). Returns true when
+// the brick ends at this step (every child stopped).
+template <typename LT, class C, class CS>
+__device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& S, LevelVars<LT>& v,
+                                           const LevelConst<LT>& c, Blocks& blk, uint32_t l, uint32_t& status) {
+  constexpr uint32_t NT = CS::NT, NW = CS::NW;
+  constexpr bool ONE = NW == 1;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const uint32_t t = c.t, P = c.P, nsym = c.nsym;
+  const uint32_t N = 1u << (3 * l), Cn = N << 3, K = 8 * v.A;
+  const bool last = (l + 1 == t);
+  uint32_t mode = MODE_LEVEL;
+  if constexpr (!ONE) mode = !last ? MODE_LEVEL : ((C::KCAP != 0 && K <= C::KCAP) ? MODE_COMPACT : MODE_GLOBAL);
+  LT* olev = c.o + (c.pyr ? fcs_pyramid_level_offset(l + 1) : 0ull);
+  uint16_t* vcur = v.vcur;
+  uint16_t* vnext = (Cn <= C::VCAP) ? (vcur == S.sA ? S.sB : S.sA) : S.big;
+  // compact op results: the free ping-pong buffer and the area next to it
+  uint16_t* opv = (vcur == S.sA) ? S.sA + C::VCAP : S.sA;
+  uint32_t* scur = v.scur;
+  uint32_t* snext = v.snext;
+  const uint32_t A = v.A;
+
+  // ---- P1: stopped children; parse and resolve the level's ops
+  if (mode == MODE_LEVEL) {
+    for (uint32_t m = tid; m < N; m += NT)
+      if (is_stopped(scur, m)) {
+        const uint32_t x = vcur[m] | ((uint32_t)vcur[m] << 16);
+        *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(x, x, x, x);
+      }
+  }
+  LevelIO<LT> io{c.nibw, v.ns, nsym, c.w_end, K, v.ip0, P, (1u << (3 * (l + 1))) - 1u, vcur, vnext, snext, opv, olev};
+  uint32_t found;
+  if constexpr (ONE) {
+    found = parse_level<MODE_LEVEL, LT, CS>(S, io, blk, v.tile, status);
+  } else {
+    if (mode == MODE_LEVEL) found = parse_level<MODE_LEVEL, LT, CS>(S, io, blk, v.tile, status);
+    else if (mode == MODE_COMPACT) found = parse_level<MODE_COMPACT, LT, CS>(S, io, blk, v.tile, status);
+    else found = parse_level<MODE_GLOBAL, LT, CS>(S, io, blk, v.tile, status);
+  }
+  if (found < K) status |= FCS_ST_STREAM_OVERRUN | FCS_ST_NSYM_MISMATCH;
+  if (mode == MODE_GLOBAL) __syncthreads(); else group_sync<CS>();  // per-node results in the cache region
+  v.ns = S.scr[SCR_NS];
+  v.ip0 = S.scr[SCR_IP];
+  if (l + 1 == c.L && v.ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
+
+  // ---- P2: chase references. A valid reference points to a smaller node,
+  // so a chain strictly decreases; anything else is corrupt input. Only op
+  // children can hold references.
+  auto act_node = [&](uint32_t k) -> uint32_t {
+    if constexpr (C::PACKED) return S.act32[k >> 3] & 0xFFFFu; else return S.act16[k >> 3];
+  };
+  uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
+  const uint32_t nw = (Cn + 31) >> 5;
+  if (mode == MODE_LEVEL) {
+    // one active parent's 8 children per step (one 16-byte LDS read)
+    for (uint32_t gp = tid; gp < A; gp += NT) {
+      const uint32_t n0 = 8 * act_node(8 * gp);
+      const uint4 q = *reinterpret_cast<const uint4*>(vnext + n0);
+      if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
+        for (uint32_t ch = 0; ch < 8; ++ch) {
+          const uint32_t n = n0 + ch;
+          uint32_t x = vnext[n];
+          if (!(x & FCS_REF16)) continue;
+          uint32_t cur = n;
+          do {
+            const uint32_t nx = x & 0x7FFFu;
+            if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
+            cur = nx;
+            x = vnext[nx];
+          } while (x & FCS_REF16);
+          vnext[n] = (uint16_t)x;
+        }
+      }
+    }
+    // active nodes of level l+1, over consecutive stop words per thread
+    const uint32_t wpt = (nw + NT - 1) / NT;
+    w0 = min(nw, tid * wpt);
+    w1 = min(nw, w0 + wpt);
+    const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
+    for (uint32_t w = w0; w < w1; ++w) cnt += __popc(~snext[w] & valid);
+    inc = wave_incl_scan(cnt);
+    if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
+  } else if (mode == MODE_COMPACT) {
+    if constexpr (!ONE) {
+      // one active parent's 8 op results per step (one 16-byte LDS read)
+      for (uint32_t gp = tid; gp < A; gp += NT) {
+        const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
+        if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
+          const uint32_t n0 = 8 * act_node(8 * gp);
+          for (uint32_t ch = 0; ch < 8; ++ch) {
+            const uint32_t k = 8 * gp + ch;
+            uint32_t x = opv[k];
+            if (!(x & FCS_REF16)) continue;
+            uint32_t cur = n0 + ch;
+            for (;;) {
+              const uint32_t nb = x & 0x7FFFu;
+              if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
+              const uint32_t p = nb >> 3;
+              if (is_stopped(scur, p)) { x = vcur[p]; break; }
+              x = opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
+              if (!(x & FCS_REF16)) break;
+              cur = nb;
+            }
+            opv[k] = (uint16_t)x;
+          }
+        }
+      }
+    }
+  } else {
+    if constexpr (!ONE) {
+      for (uint32_t k = tid; k < K; k += NT) {
+        const uint32_t n = 8 * act_node(k) + (k & 7u);
+        uint32_t x = (uint32_t)olev[n];
+        if (x & FCS_REF32) {
+          uint32_t cur = n;
+          for (;;) {
+            const uint32_t nb = x & ~FCS_REF32;
+            if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
+            const uint32_t p = nb >> 3;
+            if (is_stopped(scur, p)) { x = vcur[p]; break; }
+            x = (uint32_t)olev[nb];
+            if (!(x & FCS_REF32)) break;
+            cur = nb;
+          }
+          olev[n] = (LT)x;
+        }
+      }
+    }
+  }
+  if (mode == MODE_GLOBAL) __syncthreads(); else group_sync<CS>();
+
+  // ---- P3: next level's active list; labels out
+  bool ended = false;
+  if (mode == MODE_LEVEL) {
+    uint32_t An = 0, wb = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+      const uint32_t sw = S.scr[SCR_WAVESUM + w];
+      wb += (w < wave) ? sw : 0u;
+      An += sw;
+    }
+    if (c.pyr)
+      for (uint32_t n = tid; n < Cn; n += NT) {
+        const uint32_t x = vnext[n];
+        olev[n] = c.lp[x < P ? x : P - 1];
+      }
+    if (An == 0) {
+      // every node of level l+1 is stopped: the remaining levels are copies
+      // (the first wave alone leaves them to the whole workgroup)
+      if constexpr (!ONE) {
+        expand_rest<LT, CS>(S, c, vnext, l + 1);
+        if (t == c.L && v.ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
+      }
+      ended = true;
+    } else {
+      uint32_t k0 = wb + inc - cnt;
+      const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
+      for (uint32_t w = w0; w < w1; ++w) {
+        S.wpre[w] = (uint16_t)k0;
+        uint32_t bits = ~snext[w] & valid;
+        while (bits) {
+          const uint32_t j = __ffs(bits) - 1;
+          bits &= bits - 1;
+          const uint32_t m = w * 32 + j;
+          if constexpr (C::PACKED) S.act32[k0++] = m | ((uint32_t)vnext[m] << 16);
+          else S.act16[k0++] = (uint16_t)m;
+        }
+      }
+      if (l + 2 < t) {
+        // stop bits of level l+2: children of stopped nodes are stopped
+        for (uint32_t w = tid; w < Cn / 4; w += NT) {
+          const uint32_t pb = (snext[w >> 3] >> ((w & 7) * 4)) & 0xFu;  // 4 parents
+          uint32_t mm = 0;
+          mm |= (pb & 1u) ? 0x000000FFu : 0u;
+          mm |= (pb & 2u) ? 0x0000FF00u : 0u;
+          mm |= (pb & 4u) ? 0x00FF0000u : 0u;
+          mm |= (pb & 8u) ? 0xFF000000u : 0u;
+          scur[w] = mm;
+        }
+      }
+      if (tid == 0) { S.scr[SCR_NS] = v.ns; S.scr[SCR_IP] = v.ip0; }
+    }
+    v.vcur = vnext;
+    v.scur = snext;
+    v.snext = scur;
+    v.A = An;
+  } else if (mode == MODE_COMPACT) {
+    if constexpr (!ONE) {
+      if (c.staged) out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, S.spal, P);
+      else out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, c.pal, P);
+    }
+  } else {
+    if constexpr (!ONE) {
+      if (c.staged) out_global<LT, NT>(olev, N, scur, vcur, S.spal, P);
+      else out_global<LT, NT>(olev, N, scur, vcur, c.pal, P);
+    }
+  }
+  group_sync<CS>();
+  (void)a;
+  return ended;
+}
+
 template <typename LT, class C>
 __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t r, const DecodeReq& q,
                                const DevBrick& B) {
@@ -499,189 +766,52 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   }
   lds_barrier();
 
-  uint16_t* vcur = S.sA;
-  uint32_t* scur = S.st0;
-  uint32_t* snext = S.st1;
-  // level l's ops start at nibble ns with palette cursor ip0 (nibble 0 is the
-  // root op); each level's end is found while parsing it
-  uint32_t A = 1, ns = 1, ip0 = 0, tile = 0;
-
-  for (uint32_t l = 0; l < t; ++l) {
-    const uint32_t N = 1u << (3 * l), Cn = N << 3, K = 8 * A;
-    const bool last = (l + 1 == t);
-    const uint32_t mode = !last ? MODE_LEVEL : ((C::KCAP != 0 && K <= C::KCAP) ? MODE_COMPACT : MODE_GLOBAL);
-    LT* olev = o + (pyr ? fcs_pyramid_level_offset(l + 1) : 0ull);
-    uint16_t* vnext = (Cn <= C::VCAP) ? (vcur == S.sA ? S.sB : S.sA) : S.big;
-    // compact op results: the free ping-pong buffer and the area next to it
-    uint16_t* opv = (vcur == S.sA) ? S.sA + C::VCAP : S.sA;
-
-    // ---- P1: stopped children; parse and resolve the level's ops
-    if (mode == MODE_LEVEL) {
-      for (uint32_t m = tid; m < N; m += NT)
-        if (is_stopped(scur, m)) {
-          const uint32_t v = vcur[m] | ((uint32_t)vcur[m] << 16);
-          *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(v, v, v, v);
-        }
-    }
-    LevelIO<LT> io{nibw, ns, nsym, w_end, K, ip0, P, (1u << (3 * (l + 1))) - 1u, vcur, vnext, snext, opv, olev};
-    uint32_t found;
-    if (mode == MODE_LEVEL) found = parse_level<MODE_LEVEL, LT, C>(S, io, blk, tile, status);
-    else if (mode == MODE_COMPACT) found = parse_level<MODE_COMPACT, LT, C>(S, io, blk, tile, status);
-    else found = parse_level<MODE_GLOBAL, LT, C>(S, io, blk, tile, status);
-    if (found < K) status |= FCS_ST_STREAM_OVERRUN | FCS_ST_NSYM_MISMATCH;
-    if (mode == MODE_GLOBAL) __syncthreads(); else lds_barrier();  // per-node results in the cache region
-    ns = S.scr[SCR_NS];
-    ip0 = S.scr[SCR_IP];
-    if (l + 1 == L && ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
-
-    // ---- P2: chase references. A valid reference points to a smaller node,
-    // so a chain strictly decreases; anything else is corrupt input. Only op
-    // children can hold references.
-    auto act_node = [&](uint32_t k) -> uint32_t {
-      if constexpr (C::PACKED) return S.act32[k >> 3] & 0xFFFFu; else return S.act16[k >> 3];
-    };
-    uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
-    const uint32_t nw = (Cn + 31) >> 5;
-    if (mode == MODE_LEVEL) {
-      // one active parent's 8 children per step (one 16-byte LDS read)
-      for (uint32_t gp = tid; gp < A; gp += NT) {
-        const uint32_t n0 = 8 * act_node(8 * gp);
-        const uint4 q = *reinterpret_cast<const uint4*>(vnext + n0);
-        if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
-          for (uint32_t c = 0; c < 8; ++c) {
-            const uint32_t n = n0 + c;
-            uint32_t v = vnext[n];
-            if (!(v & FCS_REF16)) continue;
-            uint32_t cur = n;
-            do {
-              const uint32_t nx = v & 0x7FFFu;
-              if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-              cur = nx;
-              v = vnext[nx];
-            } while (v & FCS_REF16);
-            vnext[n] = (uint16_t)v;
-          }
-        }
-      }
-      // active nodes of level l+1, over consecutive stop words per thread
-      const uint32_t wpt = (nw + NT - 1) / NT;
-      w0 = min(nw, tid * wpt);
-      w1 = min(nw, w0 + wpt);
-      const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
-      for (uint32_t w = w0; w < w1; ++w) cnt += __popc(~snext[w] & valid);
-      inc = wave_incl_scan(cnt);
-      if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
-    } else if (mode == MODE_COMPACT) {
-      // one active parent's 8 op results per step (one 16-byte LDS read)
-      for (uint32_t gp = tid; gp < A; gp += NT) {
-        const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
-        if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
-          const uint32_t n0 = 8 * act_node(8 * gp);
-          for (uint32_t c = 0; c < 8; ++c) {
-            const uint32_t k = 8 * gp + c;
-            uint32_t v = opv[k];
-            if (!(v & FCS_REF16)) continue;
-            uint32_t cur = n0 + c;
-            for (;;) {
-              const uint32_t nb = v & 0x7FFFu;
-              if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-              const uint32_t p = nb >> 3;
-              if (is_stopped(scur, p)) { v = vcur[p]; break; }
-              v = opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
-              if (!(v & FCS_REF16)) break;
-              cur = nb;
-            }
-            opv[k] = (uint16_t)v;
-          }
-        }
-      }
-    } else {
-      for (uint32_t k = tid; k < K; k += NT) {
-        const uint32_t n = 8 * act_node(k) + (k & 7u);
-        uint32_t v = (uint32_t)olev[n];
-        if (v & FCS_REF32) {
-          uint32_t cur = n;
-          for (;;) {
-            const uint32_t nb = v & ~FCS_REF32;
-            if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; v = 0; break; }
-            const uint32_t p = nb >> 3;
-            if (is_stopped(scur, p)) { v = vcur[p]; break; }
-            v = (uint32_t)olev[nb];
-            if (!(v & FCS_REF32)) break;
-            cur = nb;
-          }
-          olev[n] = (LT)v;
-        }
+  // level 0 state: the root (palette index 0) is the one active node
+  LevelVars<LT> lv{S.sA, S.st0, S.st1, 1u, 1u, 0u, 0u};
+  LevelConst<LT> lc{nibw, nsym, w_end, t, L, P, pal, lp, o, pyr, staged};
+  // Non-final levels whose children fit one wave's tiles (8, 64, 512) are
+  // decoded by the first wave alone, with wave-level syncs; the other waves
+  // wait at the handoff barrier. A level whose children are all stopped ends
+  // the brick there; its expansion then runs with the whole workgroup.
+  uint32_t l = 0, ended_at = 0;
+  if (wave == 0) {
+    Blocks sb;
+    sb.bc = 0;
+    load_block<64>(nibw, w_end, 0, sb.w0, sb.n0);
+    load_block<64>(nibw, w_end, 1, sb.w1, sb.n1);
+    for (; l + 1 < t && l < 3; ++l) {
+      if (level_step<LT, C, OneWave<C>>(a, S, lv, lc, sb, l, status)) {
+        ended_at = l + 1;
+        ++l;
+        break;
       }
     }
-    if (mode == MODE_GLOBAL) __syncthreads(); else lds_barrier();
-
-    // ---- P3: next level's active list; labels out
-    bool ended = false;
-    if (mode == MODE_LEVEL) {
-      uint32_t An = 0, wb = 0;
-#pragma unroll
-      for (uint32_t w = 0; w < NW; ++w) {
-        const uint32_t s = S.scr[SCR_WAVESUM + w];
-        wb += (w < wave) ? s : 0u;
-        An += s;
-      }
-      if (pyr)
-        for (uint32_t n = tid; n < Cn; n += NT) {
-          const uint32_t v = vnext[n];
-          olev[n] = lp[v < P ? v : P - 1];
-        }
-      if (An == 0) {
-        // every node of level l+1 is stopped: the remaining levels are copies
-        for (uint32_t lt = pyr ? l + 2 : t; lt <= t; ++lt) {
-          LT* ol = o + (pyr ? fcs_pyramid_level_offset(lt) : 0ull);
-          if (staged) expand_level<LT, NT>(ol, lt, l + 1, vnext, S.spal, P);
-          else expand_level<LT, NT>(ol, lt, l + 1, vnext, pal, P);
-        }
-        if (t == L && ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
-        ended = true;
-      } else {
-        uint32_t k0 = wb + inc - cnt;
-        const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
-        for (uint32_t w = w0; w < w1; ++w) {
-          S.wpre[w] = (uint16_t)k0;
-          uint32_t bits = ~snext[w] & valid;
-          while (bits) {
-            const uint32_t j = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const uint32_t m = w * 32 + j;
-            if constexpr (C::PACKED) S.act32[k0++] = m | ((uint32_t)vnext[m] << 16);
-            else S.act16[k0++] = (uint16_t)m;
-          }
-        }
-        if (l + 2 < t) {
-          // stop bits of level l+2: children of stopped nodes are stopped
-          for (uint32_t w = tid; w < Cn / 4; w += NT) {
-            const uint32_t pb = (snext[w >> 3] >> ((w & 7) * 4)) & 0xFu;  // 4 parents
-            uint32_t mm = 0;
-            mm |= (pb & 1u) ? 0x000000FFu : 0u;
-            mm |= (pb & 2u) ? 0x0000FF00u : 0u;
-            mm |= (pb & 4u) ? 0x00FF0000u : 0u;
-            mm |= (pb & 8u) ? 0xFF000000u : 0u;
-            scur[w] = mm;
-          }
-        }
-        if (tid == 0) { S.scr[SCR_NS] = ns; S.scr[SCR_IP] = ip0; }
-      }
-      vcur = vnext;
-      uint32_t* tmp = scur;
-      scur = snext;
-      snext = tmp;
-      A = An;
-    } else if (mode == MODE_COMPACT) {
-      if (staged) out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, S.spal, P);
-      else out_compact<LT, NT>(olev, N, A, scur, S.wpre, vcur, opv, pal, P);
-    } else {
-      if (staged) out_global<LT, NT>(olev, N, scur, vcur, S.spal, P);
-      else out_global<LT, NT>(olev, N, scur, vcur, pal, P);
+    if (tid == 0) {
+      S.scr[SCR_HAND + 0] = lv.A;
+      S.scr[SCR_HAND + 1] = lv.ns;
+      S.scr[SCR_HAND + 2] = lv.ip0;
+      S.scr[SCR_HAND + 3] = l;
+      S.scr[SCR_HAND + 4] = ended_at;
     }
-    lds_barrier();
-    if (ended) break;
+  }
+  lds_barrier();
+  l = S.scr[SCR_HAND + 3];
+  ended_at = S.scr[SCR_HAND + 4];
+  lv.A = S.scr[SCR_HAND + 0];
+  lv.ns = S.scr[SCR_HAND + 1];
+  lv.ip0 = S.scr[SCR_HAND + 2];
+  lv.tile = 0;
+  // after l levels the ping-pong buffers have swapped l times
+  lv.vcur = (l & 1u) ? S.sB : S.sA;
+  lv.scur = (l & 1u) ? S.st1 : S.st0;
+  lv.snext = (l & 1u) ? S.st0 : S.st1;
+  if (ended_at) {
+    // every node of level ended_at is stopped: the remaining levels are copies
+    expand_rest<LT, C>(S, lc, lv.vcur, ended_at);
+    if (t == L && lv.ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
+  } else {
+    for (; l < t; ++l)
+      if (level_step<LT, C, C>(a, S, lv, lc, blk, l, status)) break;
   }
   if (status) atomicOr(&S.scr[SCR_STATUS], status);
   lds_barrier();
