@@ -257,11 +257,11 @@ int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& r
   if (v->label_bits == 32) {
     OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, ln.d_req,
                         (uint32_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit));
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit, c->ops_head));
   } else {
     OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, ln.d_req,
                         (uint64_t*)v->d_cache, ln.d_status, ln.d_counter, (uint32_t)n, v->nb, v->L};
-    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit));
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, stream, nullptr, ksplit, c->ops_head));
   }
   if (timed) HIP_TRY(c, hipEventRecord(c->ev[1], stream));
   ln.status_sorted.resize(n);
@@ -562,6 +562,11 @@ int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
   if (option == FCS_OPT_OPS_SPLIT) {
     if (value < FCS_OPS_SPLIT_AUTO || value > FCS_OPS_SPLIT_4WG) return fail(c, FCS_E_INVALID, "bad ops split");
     c->ops_split = (int)value;
+    return 0;
+  }
+  if (option == FCS_OPT_OPS_HEAD) {
+    if (value < FCS_OPS_HEAD_AUTO || value > FCS_OPS_HEAD_OFF) return fail(c, FCS_E_INVALID, "bad head mode");
+    c->ops_head = (int)value;
     return 0;
   }
   return fail(c, FCS_E_INVALID, "unknown option");

@@ -27,6 +27,7 @@ struct fcs_ctx {
   fcs_timing timing{};
   int rans_mapping = FCS_RANS_MAPPING_AUTO;
   int ops_split = FCS_OPS_SPLIT_AUTO;
+  int ops_head = FCS_OPS_HEAD_AUTO;
   std::vector<fcs_volume*> volumes;
   std::vector<fcs_mlp*> mlps;
 };

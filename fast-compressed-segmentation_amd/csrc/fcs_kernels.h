@@ -39,6 +39,7 @@ struct OpsArgs {
   uint32_t nreq;
   uint32_t nbricks;
   uint32_t L;
+  uint32_t head;      // set by the launcher: a head pass left levels 1..3 in the outputs
 };
 
 // per-brick frame metadata for the marcher
@@ -117,10 +118,13 @@ size_t fcs_ops_lds_bytes(uint32_t L);
 // Op decoder LDS splits for bricks up to 32^3 (64^3 bricks have one).
 #define FCS_OPS_3WG 0  // ~52 KB LDS, three workgroups per CU
 #define FCS_OPS_4WG 1  // ~37 KB LDS, four workgroups per CU
+// launches of at least this many requests decode their first levels in a head
+// pass (head: FCS_OPS_HEAD_AUTO / _ON / _OFF)
+#define FCS_OPS_HEAD_MIN 1024u
 // Persistent launch: min(nreq, resident workgroups per CU x num_cu) workgroups.
 template <typename LT>
 hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr,
-                          int split = FCS_OPS_3WG);
+                          int split = FCS_OPS_3WG, int head = 0);
 // First-hit march; with a.out_rgba set, the composited shaded march that also
 // writes the first-hit outputs.
 template <typename LT>

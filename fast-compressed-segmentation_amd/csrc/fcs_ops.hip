@@ -55,6 +55,8 @@ using CfgSmall = OpsCfg<256, 4096, 8192, 0, 4096, true, 128, 2048>;
 using CfgSmall4 = OpsCfg<256, 4096, 4096, 0, 4096, false, 128, 1024, 4>;
 // 64^3 bricks: sixteen waves, the 32768-node level in LDS (~155 KB)
 using CfgLarge = OpsCfg<1024, 4096, 0, 32768, 32768, false, 1024, 0>;
+// head pass: one wave decodes levels 1..3 of one request (~6.5 KB)
+using CfgHead = OpsCfg<64, 512, 0, 0, 512, true, 128, 0>;
 
 constexpr uint32_t SCR_WORDS = 256;
 constexpr uint32_t SCR_WAVESUM = 0;   // NW words: active-node counts per wave
@@ -66,6 +68,7 @@ constexpr uint32_t SCR_IP = 123;      // palette cursor at the next level's star
 constexpr uint32_t SCR_DUM16 = 128;   // 64 u16: write target of lanes without an op
 constexpr uint32_t SCR_DUM32 = 160;   // 64 u32: same, for stop bits
 constexpr uint32_t SCR_IDX = 224;     // 4 words: request index ring
+constexpr uint32_t SCR_HEAD = 232;    // 8 words: header of the head pass state
 
 template <class C, typename LT>
 struct Layout {
@@ -99,6 +102,25 @@ struct Lds {
   DevBrick* rb;     // brick-record ring (4)
   LT* spal;
 };
+
+template <class C, typename LT>
+__device__ __forceinline__ Lds<LT> make_lds(uint8_t* smem) {
+  using Y = Layout<C, LT>;
+  Lds<LT> S;
+  S.sA = reinterpret_cast<uint16_t*>(smem + Y::sA);
+  S.sB = reinterpret_cast<uint16_t*>(smem + Y::sB);
+  S.big = reinterpret_cast<uint16_t*>(smem + Y::big);
+  S.act32 = reinterpret_cast<uint32_t*>(smem + Y::act);
+  S.act16 = reinterpret_cast<uint16_t*>(smem + Y::act);
+  S.wpre = reinterpret_cast<uint16_t*>(smem + Y::wpre);
+  S.st0 = reinterpret_cast<uint32_t*>(smem + Y::st0);
+  S.st1 = reinterpret_cast<uint32_t*>(smem + Y::st1);
+  S.scr = reinterpret_cast<uint32_t*>(smem + Y::scr);
+  S.rq = reinterpret_cast<DecodeReq*>(smem + Y::ring);
+  S.rb = reinterpret_cast<DevBrick*>(smem + Y::ring + 64);
+  S.spal = reinterpret_cast<LT*>(smem + Y::spal);
+  return S;
+}
 
 enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2 };
 
@@ -711,16 +733,62 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   return ended;
 }
 
-template <typename LT, class C>
+// Level state after the first levels of a request, as the head pass leaves it
+// in the request's output region (the finest level's slot of a pyramid, which
+// only the request's last level writes). u32 words:
+//   0: HEAD_MAGIC | level l | ended << 4   1: active nodes A   2: next nibble
+//   3: palette cursor   4: status bits   5-7: unused
+//   then level l's values (u16 pairs), its stop bits, the next level's stop
+//   bits preset from stopped parents (when that level is not the last), the
+//   active counts before each stop word (u16 pairs), the active list
+//   (node | value << 16). At most 928 words (level 3 of a 32^3 request),
+//   within every output the head pass writes to (8^t labels, t >= 2).
+constexpr uint32_t HEAD_MAGIC = 0xC5D00000u;
+constexpr uint32_t HEAD_MAX_LEVEL = 3;
+constexpr uint32_t HEAD_WORDS = 928;
+
+struct HeadLayout {
+  uint32_t N, nvw, nsw, npw, nww;  // nodes; words of values, stop bits, preset stop bits, rank prefixes
+  uint32_t v, st, ps, wp, act;     // word offsets
+};
+
+__device__ __forceinline__ HeadLayout head_layout(uint32_t l, uint32_t t) {
+  HeadLayout h;
+  h.N = 1u << (3 * l);
+  h.nvw = (h.N + 1) / 2;
+  h.nsw = (h.N + 31) / 32;
+  h.npw = (l + 1 < t) ? (8 * h.N + 31) / 32 : 0u;
+  h.nww = (h.nsw + 1) / 2;
+  h.v = 8;
+  h.st = h.v + h.nvw;
+  h.ps = h.st + h.nsw;
+  h.wp = h.ps + h.npw;
+  h.act = h.wp + h.nww;
+  return h;
+}
+
+template <typename LT>
+__device__ __forceinline__ uint32_t* head_state(LT* o, uint32_t t, bool pyr) {
+  return reinterpret_cast<uint32_t*>(o + (pyr ? fcs_pyramid_level_offset(t) : 0ull));
+}
+
+// `commit` is called once on every path, as soon as the request's first
+// global loads have been waited for: the caller stores its prefetched ring
+// entries there, so that their wait does not queue behind this request's
+// label stores (global loads and stores complete in issue order).
+template <typename LT, class C, class Commit>
 __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t r, const DecodeReq& q,
-                               const DevBrick& B) {
+                               const DevBrick& B, Commit&& commit) {
   constexpr uint32_t NT = C::NT, NW = C::NW;
   constexpr uint32_t PCAP = Layout<C, LT>::PCAP;
+  constexpr uint32_t PPT = PCAP ? (PCAP + NT - 1) / NT : 1;  // staged palette entries per thread
+  constexpr uint32_t HW = (HEAD_WORDS + NT - 1) / NT;        // head state words per thread
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const uint32_t b = q.brick, t = q.lod & 0xFFu;
   const bool pyr = (q.lod & FCS_REQ_PYRAMID) != 0;
   const uint32_t L = a.L;
   if (b >= a.nbricks || t > L) {
+    commit();
     if (tid == 0) a.status[r] = FCS_ST_BAD_REQUEST;
     return;
   }
@@ -731,9 +799,13 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   const LT* pal = a.palette + B.pal_off;
   LT* o = a.out + q.out_off;
   if (P == 0 || P > FCS_PALETTE_MAX_LDS) {
+    commit();
     if (tid == 0) a.status[r] = status | (P == 0 ? FCS_ST_PALETTE_RANGE : FCS_ST_BAD_REQUEST);
     return;
   }
+  // Every start-up load is issued here, before any wait: the nibble blocks,
+  // the root, the staged palette and (for streams longer than the root) the
+  // head pass state, read whole from the request's output.
   const uint32_t* nibw = a.nib + B.nib_off;
   const uint32_t w_end = (nsym + 7) >> 3;
   Blocks blk;
@@ -742,15 +814,45 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   load_block<NT>(nibw, w_end, 1, blk.w1, blk.n1);
   const uint32_t root = nibw[0] & 0xFu;
   const LT p0 = pal[0];
+  const bool staged = PCAP != 0 && P <= PCAP;
+  LT pv[PPT];
+#pragma unroll
+  for (uint32_t k = 0; k < PPT; ++k) {
+    const uint32_t j = tid + k * NT;
+    pv[k] = (staged && j < P) ? pal[j] : LT(0);
+  }
+  const bool use_head = a.head && t >= 2;
+  const uint32_t* hs = head_state(o, t, pyr);
+  const uint32_t hn = min(HEAD_WORDS, (uint32_t)((sizeof(LT) / 4) << (3 * t)));
+  uint32_t hw[HW];
+  const bool pre = use_head && nsym > 1;
+#pragma unroll
+  for (uint32_t k = 0; k < HW; ++k) {
+    const uint32_t j = tid + k * NT;
+    hw[k] = (pre && j < hn) ? hs[j] : 0u;
+  }
+  commit();
   if ((root & 1u) || t == 0 || (status & FCS_ST_STREAM_OVERRUN)) {
     // constant brick (root stop bit) or LOD 0: every voxel = palette[0]
     fill_const<LT, NT>(o, pyr ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t)), p0);
     if (tid == 0) a.status[r] = status;
     return;
   }
-  const bool staged = PCAP != 0 && P <= PCAP;
-  if (staged)
-    for (uint32_t i = tid; i < P; i += NT) S.spal[i] = pal[i];
+  if (use_head && !pre) {
+    // (a one-nibble stream whose root is not stopped: corrupt, rare)
+#pragma unroll
+    for (uint32_t k = 0; k < HW; ++k) {
+      const uint32_t j = tid + k * NT;
+      hw[k] = j < hn ? hs[j] : 0u;
+    }
+  }
+  if (staged) {
+#pragma unroll
+    for (uint32_t k = 0; k < PPT; ++k) {
+      const uint32_t j = tid + k * NT;
+      if (j < P) S.spal[j] = pv[k];
+    }
+  }
   const LT* lp = staged ? static_cast<const LT*>(S.spal) : pal;
   if (pyr && tid == 0) o[0] = p0;
   if (tid == 0) {
@@ -764,6 +866,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     S.scr[SCR_NS] = 1;
     S.scr[SCR_IP] = 0;
   }
+  if (use_head && tid < 8) S.scr[SCR_HEAD + tid] = hw[0];
   lds_barrier();
 
   // level 0 state: the root (palette index 0) is the one active node
@@ -774,32 +877,71 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   // wait at the handoff barrier. A level whose children are all stopped ends
   // the brick there; its expansion then runs with the whole workgroup.
   uint32_t l = 0, ended_at = 0;
-  if (wave == 0) {
-    Blocks sb;
-    sb.bc = 0;
-    load_block<64>(nibw, w_end, 0, sb.w0, sb.n0);
-    load_block<64>(nibw, w_end, 1, sb.w1, sb.n1);
-    for (; l + 1 < t && l < 3; ++l) {
-      if (level_step<LT, C, OneWave<C>>(a, S, lv, lc, sb, l, status)) {
-        ended_at = l + 1;
-        ++l;
-        break;
-      }
+  if (use_head) {
+    // the head pass decoded the first levels: its state, held by the
+    // threads in word order, goes to the LDS arrays
+    const uint32_t h0 = S.scr[SCR_HEAD];
+    if ((h0 & 0xFFFF0000u) != HEAD_MAGIC || (h0 & 0xFu) > HEAD_MAX_LEVEL || (h0 & 0xFu) >= t ||
+        ((h0 >> 4) & 0xFu) > (h0 & 0xFu)) {
+      if (tid == 0) a.status[r] = status | FCS_ST_BAD_REQUEST;
+      return;
+    }
+    l = h0 & 0xFu;
+    ended_at = (h0 >> 4) & 0xFu;
+    const HeadLayout h = head_layout(l, t);
+    lv.A = min(S.scr[SCR_HEAD + 1], h.N);
+    lv.ns = S.scr[SCR_HEAD + 2];
+    lv.ip0 = S.scr[SCR_HEAD + 3];
+    status |= S.scr[SCR_HEAD + 4];
+    uint32_t* vc = reinterpret_cast<uint32_t*>((l & 1u) ? S.sB : S.sA);
+    uint32_t* sc = (l & 1u) ? S.st1 : S.st0;
+    uint32_t* sn = (l & 1u) ? S.st0 : S.st1;
+    uint32_t* wp = reinterpret_cast<uint32_t*>(S.wpre);
+    const uint32_t end = min(h.act + lv.A, hn);
+#pragma unroll
+    for (uint32_t k = 0; k < HW; ++k) {
+      const uint32_t j = tid + k * NT, w = hw[k];
+      if (j < h.v || j >= end) continue;
+      if (j < h.st) vc[j - h.v] = w;
+      else if (j < h.ps) sc[j - h.st] = w;
+      else if (j < h.wp) sn[j - h.ps] = w;
+      else if (j < h.act) wp[j - h.wp] = w;
+      else if constexpr (C::PACKED) S.act32[j - h.act] = w;
+      else S.act16[j - h.act] = (uint16_t)w;
     }
     if (tid == 0) {
-      S.scr[SCR_HAND + 0] = lv.A;
-      S.scr[SCR_HAND + 1] = lv.ns;
-      S.scr[SCR_HAND + 2] = lv.ip0;
-      S.scr[SCR_HAND + 3] = l;
-      S.scr[SCR_HAND + 4] = ended_at;
+      S.scr[SCR_NS] = lv.ns;
+      S.scr[SCR_IP] = lv.ip0;
     }
+    lds_barrier();
+  } else {
+    if (wave == 0) {
+      Blocks sb;
+      sb.bc = 0;
+      load_block<64>(nibw, w_end, 0, sb.w0, sb.n0);
+      load_block<64>(nibw, w_end, 1, sb.w1, sb.n1);
+      for (; l + 1 < t && l < 3; ++l) {
+        if (level_step<LT, C, OneWave<C>>(a, S, lv, lc, sb, l, status)) {
+          ended_at = l + 1;
+          ++l;
+          break;
+        }
+      }
+      if (tid == 0) {
+        S.scr[SCR_HAND + 0] = lv.A;
+        S.scr[SCR_HAND + 1] = lv.ns;
+        S.scr[SCR_HAND + 2] = lv.ip0;
+        S.scr[SCR_HAND + 3] = l;
+        S.scr[SCR_HAND + 4] = ended_at;
+      }
+    }
+    lds_barrier();
+    l = S.scr[SCR_HAND + 3];
+    ended_at = S.scr[SCR_HAND + 4];
+    lv.A = S.scr[SCR_HAND + 0];
+    lv.ns = S.scr[SCR_HAND + 1];
+    lv.ip0 = S.scr[SCR_HAND + 2];
   }
-  lds_barrier();
-  l = S.scr[SCR_HAND + 3];
-  ended_at = S.scr[SCR_HAND + 4];
-  lv.A = S.scr[SCR_HAND + 0];
-  lv.ns = S.scr[SCR_HAND + 1];
-  lv.ip0 = S.scr[SCR_HAND + 2];
   lv.tile = 0;
   // after l levels the ping-pong buffers have swapped l times
   lv.vcur = (l & 1u) ? S.sB : S.sA;
@@ -818,6 +960,76 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   if (tid == 0) a.status[r] = S.scr[SCR_STATUS];
 }
 
+// Head pass: one wave per request decodes the non-final levels up to level 3
+// (8, 64 and 512 nodes) and leaves the level state in the request's output
+// (head_state). These levels are short chains of dependent steps; run here
+// at a wave's LDS footprint, many requests overlap on every CU, where inside
+// the op kernel they would hold a whole workgroup's LDS. Requests the op
+// kernel rejects or finishes without levels (constant bricks, t < 2) are
+// skipped with the same tests.
+template <typename LT>
+__global__ __launch_bounds__(64) void ops_head_kernel(OpsArgs<LT> a) {
+  using C = CfgHead;
+  extern __shared__ __align__(16) uint8_t smem[];
+  const Lds<LT> S = make_lds<C, LT>(smem);
+  const uint32_t tid = threadIdx.x;
+  const DecodeReq q = a.reqs[blockIdx.x];
+  const uint32_t b = q.brick, t = q.lod & 0xFFu;
+  const bool pyr = (q.lod & FCS_REQ_PYRAMID) != 0;
+  if (b >= a.nbricks || t > a.L || t < 2) return;
+  const DevBrick B = a.bricks[b];
+  const uint32_t nsym = B.nsym, P = B.P;
+  if (nsym == 0 || P == 0 || P > FCS_PALETTE_MAX_LDS) return;
+  const uint32_t* nibw = a.nib + B.nib_off;
+  const uint32_t w_end = (nsym + 7) >> 3;
+  Blocks sb;
+  sb.bc = 0;
+  load_block<64>(nibw, w_end, 0, sb.w0, sb.n0);
+  load_block<64>(nibw, w_end, 1, sb.w1, sb.n1);
+  if (nibw[0] & 1u) return;  // constant brick
+  const LT* pal = a.palette + B.pal_off;
+  LT* o = a.out + q.out_off;
+  if (tid == 0) {
+    S.sA[0] = 0;
+    S.act32[0] = 0;
+    S.wpre[0] = 0;
+    S.st0[0] = 0;
+    S.st1[0] = 0;
+    S.scr[SCR_STATUS] = 0;
+    S.scr[SCR_NS] = 1;
+    S.scr[SCR_IP] = 0;
+  }
+  group_sync<C>();
+  LevelVars<LT> lv{S.sA, S.st0, S.st1, 1u, 1u, 0u, 0u};
+  LevelConst<LT> lc{nibw, nsym, w_end, t, a.L, P, pal, pal, o, pyr, false};
+  uint32_t status = 0, l = 0, ended_at = 0;
+  for (; l + 1 < t && l < HEAD_MAX_LEVEL; ++l) {
+    if (level_step<LT, C, C>(a, S, lv, lc, sb, l, status)) {
+      ended_at = l + 1;
+      ++l;
+      break;
+    }
+  }
+  if (status) atomicOr(&S.scr[SCR_STATUS], status);
+  group_sync<C>();
+  uint32_t* hs = head_state(o, t, pyr);
+  const HeadLayout h = head_layout(l, t);
+  if (tid == 0) {
+    hs[0] = HEAD_MAGIC | l | (ended_at << 4);
+    hs[1] = lv.A;
+    hs[2] = lv.ns;
+    hs[3] = lv.ip0;
+    hs[4] = S.scr[SCR_STATUS];
+  }
+  const uint32_t* vc = reinterpret_cast<const uint32_t*>(lv.vcur);
+  const uint32_t* wp = reinterpret_cast<const uint32_t*>(S.wpre);
+  for (uint32_t i = tid; i < h.nvw; i += 64) hs[h.v + i] = vc[i];
+  for (uint32_t i = tid; i < h.nsw; i += 64) hs[h.st + i] = lv.scur[i];
+  for (uint32_t i = tid; i < h.npw; i += 64) hs[h.ps + i] = lv.snext[i];
+  for (uint32_t i = tid; i < h.nww; i += 64) hs[h.wp + i] = wp[i];
+  for (uint32_t i = tid; i < lv.A; i += 64) hs[h.act + i] = S.act32[i];
+}
+
 // Persistent workgroups drain the request list, which the host orders by
 // estimated cost, heaviest first. The first half is dealt round-robin (equal
 // shares, since neighbouring requests cost about the same); the rest is
@@ -831,20 +1043,7 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
 template <typename LT, class C>
 __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
-  using Y = Layout<C, LT>;
-  Lds<LT> S;
-  S.sA = reinterpret_cast<uint16_t*>(smem + Y::sA);
-  S.sB = reinterpret_cast<uint16_t*>(smem + Y::sB);
-  S.big = reinterpret_cast<uint16_t*>(smem + Y::big);
-  S.act32 = reinterpret_cast<uint32_t*>(smem + Y::act);
-  S.act16 = reinterpret_cast<uint16_t*>(smem + Y::act);
-  S.wpre = reinterpret_cast<uint16_t*>(smem + Y::wpre);
-  S.st0 = reinterpret_cast<uint32_t*>(smem + Y::st0);
-  S.st1 = reinterpret_cast<uint32_t*>(smem + Y::st1);
-  S.scr = reinterpret_cast<uint32_t*>(smem + Y::scr);
-  S.rq = reinterpret_cast<DecodeReq*>(smem + Y::ring);
-  S.rb = reinterpret_cast<DevBrick*>(smem + Y::ring + 64);
-  S.spal = reinterpret_cast<LT*>(smem + Y::spal);
+  const Lds<LT> S = make_lds<C, LT>(smem);
   const uint32_t nreq = a.nreq;
   const DecodeReq none{0xFFFFFFFFu, 0u, 0ull};
   const DevBrick nobrick{0ull, 0ull, 0u, 0u, 0u, 0u};
@@ -887,12 +1086,15 @@ __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
       q2 = fetch_req(S.scr[SCR_IDX + ((i + 2) & 3u)]);
       b1 = fetch_brick(S.rq[(i + 1) & 3u]);
     }
-    decode_request<LT, C>(a, S, r, q, B);
-    if (threadIdx.x == 0) {
-      S.scr[SCR_IDX + ((i + 3) & 3u)] = r3;
-      S.rq[(i + 2) & 3u] = q2;
-      S.rb[(i + 1) & 3u] = b1;
-    }
+    // the ring slots written here are not read before the next iteration
+    auto commit = [&]() {
+      if (threadIdx.x == 0) {
+        S.scr[SCR_IDX + ((i + 3) & 3u)] = r3;
+        S.rq[(i + 2) & 3u] = q2;
+        S.rb[(i + 1) & 3u] = b1;
+      }
+    };
+    decode_request<LT, C>(a, S, r, q, B, commit);
     lds_barrier();
   }
 }
@@ -903,7 +1105,7 @@ __global__ void zero_u32_kernel(uint32_t* p, uint64_t n) {
 }
 
 template <typename LT, class C>
-hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out) {
+hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int head) {
   constexpr size_t lds = Layout<C, LT>::bytes;
   static_assert(lds <= 160 * 1024, "op decoder LDS budget");
   // resident workgroups per CU (queried once per instantiation)
@@ -920,8 +1122,14 @@ hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint
   const uint64_t resident = (uint64_t)per_cu * (num_cu ? num_cu : 1);
   const uint32_t grid = (uint32_t)(a.nreq < resident ? a.nreq : resident);
   if (grid_out) *grid_out = grid;
+  OpsArgs<LT> b = a;
+  b.head = head == FCS_OPS_HEAD_ON || (head == FCS_OPS_HEAD_AUTO && a.nreq >= FCS_OPS_HEAD_MIN) ? 1u : 0u;
+  if (b.head) {
+    constexpr size_t hl = Layout<CfgHead, LT>::bytes;
+    hipLaunchKernelGGL(ops_head_kernel<LT>, dim3(a.nreq), dim3(64), hl, s, b);
+  }
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, a.counter, (uint64_t)1);
-  hipLaunchKernelGGL((ops_kernel<LT, C>), dim3(grid), dim3(C::NT), lds, s, a);
+  hipLaunchKernelGGL((ops_kernel<LT, C>), dim3(grid), dim3(C::NT), lds, s, b);
   return hipGetLastError();
 }
 
@@ -932,12 +1140,13 @@ size_t fcs_ops_lds_bytes(uint32_t L) {
 }
 
 template <typename LT>
-hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int split) {
+hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int split,
+                          int head) {
   if (a.nreq == 0) return hipSuccess;
   if (a.L > 6) return hipErrorInvalidValue;  // 16-bit node indices: bricks up to 64^3
-  if (a.L == 6) return launch_cfg<LT, CfgLarge>(a, num_cu, s, grid_out);
-  if (split == FCS_OPS_4WG) return launch_cfg<LT, CfgSmall4>(a, num_cu, s, grid_out);
-  return launch_cfg<LT, CfgSmall>(a, num_cu, s, grid_out);
+  if (a.L == 6) return launch_cfg<LT, CfgLarge>(a, num_cu, s, grid_out, head);
+  if (split == FCS_OPS_4WG) return launch_cfg<LT, CfgSmall4>(a, num_cu, s, grid_out, head);
+  return launch_cfg<LT, CfgSmall>(a, num_cu, s, grid_out, head);
 }
 
 hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
@@ -946,5 +1155,5 @@ hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*, int);
-template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*, int);
+template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*, int, int);
+template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*, int, int);

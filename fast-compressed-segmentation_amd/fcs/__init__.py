@@ -11,6 +11,9 @@ from ._lib import (  # noqa: F401
     FcsError,
     lib,
     lib_path,
+    OPS_HEAD_AUTO,
+    OPS_HEAD_OFF,
+    OPS_HEAD_ON,
     ST_BAD_OP,
     ST_BAD_REQUEST,
     ST_CACHE_FULL,

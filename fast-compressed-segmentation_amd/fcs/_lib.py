@@ -123,6 +123,8 @@ RANS_AUTO, RANS_LANE, RANS_WAVE, RANS_WAVE_BALLOT, RANS_WAVE_LDS, RANS_WAVE_WIN,
 OPT_RANS_MAPPING = 1
 OPT_OPS_SPLIT = 2
 OPS_SPLIT_AUTO, OPS_SPLIT_3WG, OPS_SPLIT_4WG = 0, 1, 2
+OPT_OPS_HEAD = 3
+OPS_HEAD_AUTO, OPS_HEAD_ON, OPS_HEAD_OFF = 0, 1, 2
 
 
 class GenParams(ctypes.Structure):

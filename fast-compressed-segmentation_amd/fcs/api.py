@@ -104,6 +104,11 @@ class Context:
         1 = three workgroups per CU, 2 = four workgroups per CU."""
         check(lib.fcs_set_option(self.h, 2, int(split)), self.h)
 
+    def set_ops_head(self, mode):
+        """Levels up to 3 of each request in a separate one-wave-per-request pass: 0 = for launches of at
+        least 1024 requests, 1 = always, 2 = never."""
+        check(lib.fcs_set_option(self.h, 3, int(mode)), self.h)
+
     def sync(self):
         check(lib.fcs_sync(self.h), self.h)
 

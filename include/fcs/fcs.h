@@ -156,6 +156,11 @@ typedef struct {
 #define FCS_OPS_SPLIT_AUTO 0 /* time both splits on a volume's first batch of >= 2048 requests, keep the faster */
 #define FCS_OPS_SPLIT_3WG 1 /* op decoder LDS split for three workgroups per CU (bricks up to 32^3) */
 #define FCS_OPS_SPLIT_4WG 2 /* op decoder LDS split for four workgroups per CU (bricks up to 32^3) */
+#define FCS_OPT_OPS_HEAD 3 /* FCS_OPS_HEAD_*; default AUTO */
+#define FCS_OPS_HEAD_AUTO 0 /* decode the levels up to 3 in a separate one-wave-per-request pass for launches of
+                               >= 1024 requests */
+#define FCS_OPS_HEAD_ON 1   /* always use that pass */
+#define FCS_OPS_HEAD_OFF 2  /* never: the op decoder's first wave decodes those levels */
 
 int fcs_create(int hip_device, fcs_ctx** out);
 void fcs_destroy(fcs_ctx* ctx);
