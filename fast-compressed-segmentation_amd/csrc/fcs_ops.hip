@@ -243,12 +243,12 @@ __device__ __forceinline__ void expand_level(LT* o, uint32_t lt, uint32_t lv, co
     uint4* o4 = reinterpret_cast<uint4*>(o);
     for (uint32_t i = tid; i < n / PER; i += NT) {
       const uint32_t v = vl[(i * PER) >> shift];
-      o4[i] = splat16(pal[v < P ? v : P - 1]);
+      o4[i] = splat16(pal[min(v, P - 1)]);
     }
   } else {
     for (uint32_t i = tid; i < n; i += NT) {
       const uint32_t v = vl[i >> shift];
-      o[i] = pal[v < P ? v : P - 1];
+      o[i] = pal[min(v, P - 1)];
     }
   }
 }
@@ -265,15 +265,25 @@ __device__ __forceinline__ void out_compact(LT* olev, uint32_t N, uint32_t A, co
   for (uint32_t m = threadIdx.x; m < N; m += NT) {
     const uint32_t sw = scur[m >> 5];
     const bool stp = (sw >> (m & 31)) & 1u;
+    const uint32_t pv = vcur[m];
+    LT v8[8];
+    if (__ballot(!stp) == 0) {
+      // the wave's 64 parents are all stopped: one label per parent
+      const LT u = pl[min(pv, P - 1)];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) v8[c] = u;
+      store8(olev + 8 * m, v8, al);
+      continue;
+    }
     const uint32_t rk = min(wpre[m >> 5] + __popc(~sw & ((1u << (m & 31)) - 1u)), A - 1);
     const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * rk);
-    const uint32_t pv = vcur[m];
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-    LT v8[8];
+    // a stopped parent's children all take its value: select per word, not per label
+    const uint32_t pv2 = pv | (pv << 16);
+    const uint32_t w[4] = {stp ? pv2 : q.x, stp ? pv2 : q.y, stp ? pv2 : q.z, stp ? pv2 : q.w};
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const uint32_t v = stp ? pv : (w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
-      v8[c] = pl[v < P ? v : P - 1];
+      const uint32_t v = (c & 1) ? (w[c >> 1] >> 16) : (w[c >> 1] & 0xFFFFu);
+      v8[c] = pl[min(v, P - 1)];
     }
     store8(olev + 8 * m, v8, al);
   }
@@ -288,14 +298,14 @@ __device__ __forceinline__ void out_global(LT* olev, uint32_t N, const uint32_t*
     LT v8[8];
     if (is_stopped(scur, m)) {
       const uint32_t v = vcur[m];
-      const LT u = pl[v < P ? v : P - 1];
+      const LT u = pl[min(v, P - 1)];
 #pragma unroll
       for (int c = 0; c < 8; ++c) v8[c] = u;
     } else {
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const uint32_t v = (uint32_t)olev[8 * m + c];
-        v8[c] = pl[v < P ? v : P - 1];
+        v8[c] = pl[min(v, P - 1)];
       }
     }
     store8(olev + 8 * m, v8, al);
@@ -411,11 +421,11 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
     // live op write to a private dummy slot. Waves without a live op skip.
     const uint32_t my_ops = (myX ? v1 : v0) & 0xFFFFu;
     if (__ballot(my_ops != 0 && op_base + (ex & 0xFFFFu) < K) != 0) {
-      uint32_t k = op_base + (ex & 0xFFFFu);
-      uint32_t ipx = io.ip0 + pa_base + (ex >> 16);
+      const uint32_t k0 = op_base + (ex & 0xFFFFu), ip_start = io.ip0 + pa_base + (ex >> 16);
+      uint32_t k = k0, ipx = ip_start;
       const uint32_t iso = par ? iso1 : iso0;
-      bool has_end = false;
-      uint32_t end_g = 0, end_ip = 0;
+      // node references: bit 15 of the LDS u16 forms, bit 31 of the labels
+      constexpr uint32_t REFB = (MODE == MODE_GLOBAL) ? FCS_REF32 : FCS_REF16;
       uint16_t* dum16 = reinterpret_cast<uint16_t*>(S.scr + SCR_DUM16) + lane;
       uint32_t* dum32 = S.scr + SCR_DUM32 + lane;
 #pragma unroll 2
@@ -449,39 +459,44 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         asm volatile("" : "+v"(nv));
         // outside the brick the reference reads the parent's output slot,
         // which child 0 has already overwritten when c > 0
-        const uint32_t rval = outside ? ((c == 0) ? pv : (FCS_REF32 | (8 * m)))
-                                      : ((nb < n) ? (FCS_REF32 | nb) : nv);
+        const uint32_t rval = outside ? ((c == 0) ? pv : (REFB | (8 * m))) : ((nb < n) ? (REFB | nb) : nv);
         // Pa / P0 / Pd: palette cursor (Pd: delta nibble dn selects ipx - dn - 1)
         ipx += (live && op == FCS_OP_PA) ? 1u : 0u;
         const bool isd = (op == FCS_OP_PD);
         const uint32_t sub = isd ? (((j < 7) ? (word >> (4 * (j + 1))) : wnext) & 0xFu) + 1u : 0u;
-        const uint32_t pidx = ipx - sub;
-        const bool bad_pal = (ipx < sub) || (pidx >= P);
-        const uint32_t pval = bad_pal ? P - 1 : pidx;
+        const uint32_t pidx = ipx - sub;  // wraps past P when the delta reaches before the palette
+        const bool bad_pal = pidx >= P;
+        const uint32_t pval = min(pidx, P - 1);
         uint32_t val = (op == FCS_OP_RP) ? pv : (op <= FCS_OP_RZ) ? rval : (op <= FCS_OP_PD) ? pval : 0u;
         uint32_t sb = (op >= FCS_OP_PA && op <= FCS_OP_PD && bad_pal) ? FCS_ST_PALETTE_RANGE : 0u;
         sb |= (isd && g + 1 >= ne) ? FCS_ST_STREAM_OVERRUN : 0u;
         sb |= (op > FCS_OP_PD) ? FCS_ST_BAD_OP : 0u;
         st |= live ? sb : 0u;
-        const uint32_t v16 = (val & FCS_REF32) ? (FCS_REF16 | (val & 0x7FFFu)) : val;
         if constexpr (MODE == MODE_LEVEL) {
-          *(live ? &io.vnext[n] : dum16) = (uint16_t)v16;
+          *(live ? &io.vnext[n] : dum16) = (uint16_t)val;
           atomicOr(live ? &io.snext[n >> 5] : dum32, (live && (v & 1u)) ? (1u << (n & 31)) : 0u);
         } else if constexpr (MODE == MODE_COMPACT) {
-          *(live ? &io.opv[k] : dum16) = (uint16_t)v16;
+          *(live ? &io.opv[k] : dum16) = (uint16_t)val;
         } else {
           if (live) io.olev[n] = (LT)val;
         }
-        const bool is_end = live && k == K - 1;
-        end_g = is_end ? g + 1 + (isd ? 1u : 0u) : end_g;
-        end_ip = is_end ? ipx : end_ip;
-        has_end = has_end || is_end;
         k += isop ? 1u : 0u;
       }
-      if (has_end) {
-        // last op of the level: the next level starts after it (and its delta)
-        S.scr[SCR_NS] = end_g;
-        S.scr[SCR_IP] = end_ip;
+      if (k0 < K && k >= K) {
+        // this lane holds the level's last op: the next level starts after it
+        // (and its delta); the palette cursor there
+        uint32_t kq = k0, ipq = ip_start;
+        for (uint32_t j = 0; j < 8; ++j) {
+          if (!((iso >> j) & 1u)) continue;
+          const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
+          ipq += (op == FCS_OP_PA) ? 1u : 0u;
+          if (kq == K - 1) {
+            S.scr[SCR_NS] = g0 + j + 1 + ((op == FCS_OP_PD) ? 1u : 0u);
+            S.scr[SCR_IP] = ipq;
+            break;
+          }
+          ++kq;
+        }
       }
     }
     op_base += base & 0xFFFFu;
@@ -675,7 +690,7 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
     if (c.pyr)
       for (uint32_t n = tid; n < Cn; n += NT) {
         const uint32_t x = vnext[n];
-        olev[n] = c.lp[x < P ? x : P - 1];
+        olev[n] = c.lp[min(x, P - 1)];
       }
     if (An == 0) {
       // every node of level l+1 is stopped: the remaining levels are copies
