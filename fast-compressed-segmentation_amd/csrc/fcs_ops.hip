@@ -600,24 +600,31 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
   const uint32_t nw = (Cn + 31) >> 5;
   if (mode == MODE_LEVEL) {
-    // one active parent's 8 children per step (one 16-byte LDS read)
+    // one active parent's 8 children per step: one 16-byte LDS read, the
+    // references chased from registers, one 16-byte write back (a chase
+    // reading a group another lane rewrites sees either of two values that
+    // lead to the same label)
     for (uint32_t gp = tid; gp < A; gp += NT) {
       const uint32_t n0 = 8 * act_node(8 * gp);
       const uint4 q = *reinterpret_cast<const uint4*>(vnext + n0);
       if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
+        uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
         for (uint32_t ch = 0; ch < 8; ++ch) {
-          const uint32_t n = n0 + ch;
-          uint32_t x = vnext[n];
-          if (!(x & FCS_REF16)) continue;
-          uint32_t cur = n;
-          do {
-            const uint32_t nx = x & 0x7FFFu;
-            if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
-            cur = nx;
-            x = vnext[nx];
-          } while (x & FCS_REF16);
-          vnext[n] = (uint16_t)x;
+          const uint32_t sh = 16 * (ch & 1);
+          uint32_t x = (w[ch >> 1] >> sh) & 0xFFFFu;
+          if (x & FCS_REF16) {
+            uint32_t cur = n0 + ch;
+            do {
+              const uint32_t nx = x & 0x7FFFu;
+              if (nx >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
+              cur = nx;
+              x = vnext[nx];
+            } while (x & FCS_REF16);
+            w[ch >> 1] = (w[ch >> 1] & ~(0xFFFFu << sh)) | (x << sh);
+          }
         }
+        *reinterpret_cast<uint4*>(vnext + n0) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
     // active nodes of level l+1, over consecutive stop words per thread
@@ -630,27 +637,36 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
     if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
   } else if (mode == MODE_COMPACT) {
     if constexpr (!ONE) {
-      // one active parent's 8 op results per step (one 16-byte LDS read)
+      // one active parent's 8 op results per step: one 16-byte LDS read, the
+      // references among them chased from registers, one 16-byte write back.
+      // A chase may read a group another lane is rewriting: either value
+      // (the reference or what it resolves to) leads to the same label.
       for (uint32_t gp = tid; gp < A; gp += NT) {
         const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
         if ((q.x | q.y | q.z | q.w) & 0x80008000u) {
           const uint32_t n0 = 8 * act_node(8 * gp);
+          uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
           for (uint32_t ch = 0; ch < 8; ++ch) {
-            const uint32_t k = 8 * gp + ch;
-            uint32_t x = opv[k];
-            if (!(x & FCS_REF16)) continue;
-            uint32_t cur = n0 + ch;
-            for (;;) {
-              const uint32_t nb = x & 0x7FFFu;
-              if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
-              const uint32_t p = nb >> 3;
-              if (is_stopped(scur, p)) { x = vcur[p]; break; }
-              x = opv[8 * rank_of(scur, S.wpre, p) + (nb & 7u)];
-              if (!(x & FCS_REF16)) break;
-              cur = nb;
+            const uint32_t sh = 16 * (ch & 1);
+            uint32_t x = (w[ch >> 1] >> sh) & 0xFFFFu;
+            if (x & FCS_REF16) {
+              uint32_t cur = n0 + ch;
+              for (;;) {
+                const uint32_t nb = x & 0x7FFFu;
+                if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
+                const uint32_t p = nb >> 3;
+                const uint32_t sw = scur[p >> 5];
+                if ((sw >> (p & 31)) & 1u) { x = vcur[p]; break; }
+                const uint32_t rk = S.wpre[p >> 5] + __popc(~sw & ((1u << (p & 31)) - 1u));
+                x = opv[8 * rk + (nb & 7u)];
+                if (!(x & FCS_REF16)) break;
+                cur = nb;
+              }
+              w[ch >> 1] = (w[ch >> 1] & ~(0xFFFFu << sh)) | (x << sh);
             }
-            opv[k] = (uint16_t)x;
           }
+          *reinterpret_cast<uint4*>(opv + 8 * gp) = make_uint4(w[0], w[1], w[2], w[3]);
         }
       }
     }
@@ -1046,15 +1062,21 @@ __global__ __launch_bounds__(64) void ops_head_kernel(OpsArgs<LT> a) {
 }
 
 // Persistent workgroups drain the request list, which the host orders by
-// estimated cost, heaviest first. The first half is dealt round-robin (equal
-// shares, since neighbouring requests cost about the same); the rest is
-// claimed through one atomic counter to even out the finishing times. (One
-// counter shared by every workgroup serialises: claiming every request
-// through it caps a launch of cheap requests at the counter's atomic rate.) Request
-// metadata is fetched ahead through 4-entry rings in LDS so that its latency
-// overlaps the previous request's decode: while request i decodes, thread 0
-// takes the index of request i+3, loads the DecodeReq of request i+2 and the
-// brick record of request i+1; they are stored to LDS at the end of request i.
+// estimated cost, heaviest first. Claims are numbered: the first half is dealt
+// round-robin (equal shares, since neighbouring requests cost about the same);
+// the rest is claimed through one atomic counter to even out the finishing
+// times. (One counter shared by every workgroup serialises: claiming every
+// request through it caps a launch of cheap requests at the counter's atomic
+// rate.) Claims map to requests alternately from the heavy front and the light
+// back of the list, one grid-sized chunk at a time, so every workgroup mixes
+// compute-bound requests with cheap ones -- constant bricks are pure label
+// stores, and all of them together at the end would leave the launch's tail
+// bound by the write bandwidth with nothing to overlap. Request metadata is
+// fetched ahead through 4-entry rings in LDS so that its latency overlaps the
+// previous request's decode: while request i decodes, thread 0 takes the
+// index of request i+3, loads the DecodeReq of request i+2 and the brick
+// record of request i+1; they are stored to LDS as soon as the first loads of
+// request i have arrived.
 template <typename LT, class C>
 __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -1065,10 +1087,17 @@ __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   const uint32_t G = gridDim.x;
   const uint32_t n_static = (nreq / 2) / G * G;
   uint32_t sj = 0;  // thread 0: round-robin requests taken so far
+  // claim c -> request: chunk c / G even takes the next G requests from the
+  // front, odd from the back (a bijection on [0, nreq))
+  auto order = [&](uint32_t c) -> uint32_t {
+    if (c >= nreq) return c;
+    const uint32_t qc = c / G, pos = (qc >> 1) * G + (c - qc * G);
+    return (qc & 1u) ? nreq - 1u - pos : pos;
+  };
   auto next_index = [&]() -> uint32_t {
     const uint32_t rr = blockIdx.x + sj * G;
-    if (rr < n_static) { ++sj; return rr; }
-    return n_static + atomicAdd(a.counter, 1u);
+    if (rr < n_static) { ++sj; return order(rr); }
+    return order(n_static + atomicAdd(a.counter, 1u));
   };
   auto fetch_req = [&](uint32_t i) -> DecodeReq { return i < nreq ? a.reqs[i] : none; };
   auto fetch_brick = [&](const DecodeReq& q) -> DevBrick { return q.brick < a.nbricks ? a.bricks[q.brick] : nobrick; };
