@@ -58,6 +58,7 @@ struct fcs_volume {
   uint32_t* d_nib = nullptr;
   RansResult* d_rres = nullptr;
   uint32_t* d_rtab = nullptr;  // x -> (symbol << 25) | (freq << 12) | (x - cum)
+  uint32_t* d_rlist = nullptr; // bricks by nibble count, longest first (the rANS launch order)
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
   int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
@@ -707,6 +708,16 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     }
     if (int rc2 = dev_upload(c, &v->d_rtab, tab.data(), tab.size() * 4)) { fcs_volume_free(v); return rc2; }
   }
+  {
+    // rANS launch order: longest streams first, so that a wave's lanes
+    // decode streams of about the same length (a wave runs as long as its
+    // longest lane) and the longest ones start first
+    std::vector<uint32_t> order(v->nb);
+    for (uint32_t i = 0; i < v->nb; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return v->hb[x].nsym > v->hb[y].nsym; });
+    if (int rc2 = dev_upload(c, &v->d_rlist, order.data(), order.size() * 4)) { fcs_volume_free(v); return rc2; }
+  }
   v->pal_host = std::move(hp);
   {
     static std::atomic<uint64_t> next_serial{1};
@@ -781,7 +792,7 @@ void fcs_volume_release_device(fcs_volume* v) {
   (void)hipSetDevice(c->device);
   (void)settle_prefetch(v);
   (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab,
+  void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab, v->d_rlist,
                   v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
                   v->d_misses, v->d_csum, v->d_res, v->d_tf_keys, v->d_tf_rgba, v->d_frgba};
   for (void* q : ptrs)
@@ -828,7 +839,7 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   RansArgs a{};
   a.bricks = v->d_bricks;
   a.stream = v->d_stream;
-  a.list = nullptr;
+  a.list = v->d_rlist;
   a.nlist = v->nb;
   a.L = v->L;
   memcpy(a.freq, v->freq, sizeof a.freq);

@@ -315,77 +315,106 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
 }
 
 // ---------------------------------------------------------- lane per brick
-// Backward byte reader: 128 bytes (8 blocks of 16) are fetched per refill and
-// consumed through register shifts, so a lane waits for memory once per 128
-// stream bytes (~300 symbols) and the other waves of the SIMD fill that wait.
-struct LaneReader {
-  const uint4* base;
-  int32_t q;       // next block index to fetch (decreasing)
-  uint32_t cur;    // current word, next byte in bits 31..24
-  uint32_t nb;     // bytes left in cur
-  uint32_t w2, w1, w0, nw;  // lower words of the current block (w2 next), count
-  uint32_t nblk;   // blocks left in b0..b7
-  uint4 b0, b1, b2, b3, b4, b5, b6, b7;
-  __device__ __forceinline__ static uint32_t pick(const uint4& v, uint32_t j) {
-    return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
-  }
+// Each lane decodes one brick. Its stream is read backwards through a ring of
+// LR_BLOCKS 16-byte blocks in LDS and two words in registers (cur, the next
+// bytes; nxt, the word after it, read from the ring ahead of use). Global
+// memory is touched only every LR_CHECK symbols, an iteration all lanes of the
+// wave share: the blocks loaded at the previous checkpoint go into the ring,
+// the next ones are requested, and finished nibble words are stored. Waits for
+// global memory (which complete in issue order, per wave) therefore only ever
+// cover operations issued a whole interval earlier; a lane refilling its
+// words at its own moment waits for LDS only. A valid stream consumes at most
+// 2 bytes per symbol (the state never falls below 2^11 before renormalising),
+// so a ring holding two intervals never runs dry; a corrupt one that does
+// continues with direct byte loads, so every stream decodes exactly as the
+// sequential rule reads it.
+constexpr uint32_t LANE_NT = 256;
+constexpr uint32_t LR_BLOCKS = 4;             // 64 bytes per lane
+constexpr uint32_t LR_WORDS = LR_BLOCKS * 4;
+constexpr uint32_t LR_CHECK = 16;                  // symbols between checkpoints (<= 32 bytes)
+
+struct LaneStream {
+  const uint4* base;  // the brick's stream as 16-byte blocks
+  const uint8_t* bytes;
+  uint32_t* ring;     // LDS, LR_WORDS rows of LANE_NT words, this lane's column
+  int32_t q;          // next block to request (decreasing; below 0 reads as zeros)
+  uint32_t rd, wr;    // ring words: next unread / written so far (consumption order)
+  uint32_t cur, nb;   // current word (next byte in bits 31..24), bytes left in it
+  uint32_t pre;       // ring word rd, read at the end of the previous symbol
+  bool pre_ok, staged, direct;
+  uint4 s0, s1;       // blocks requested at the last checkpoint
+
   __device__ __forceinline__ uint4 blk(int32_t k) const {
     return k >= 0 ? base[k] : make_uint4(0u, 0u, 0u, 0u);
   }
-  __device__ __forceinline__ void refill() {
-    b0 = blk(q);
-    b1 = blk(q - 1);
-    b2 = blk(q - 2);
-    b3 = blk(q - 3);
-    b4 = blk(q - 4);
-    b5 = blk(q - 5);
-    b6 = blk(q - 6);
-    b7 = blk(q - 7);
-    q -= 8;
-    nblk = 8;
+  // consumption word d: block (first - d / 4), word 3 - d % 4 of it
+  __device__ __forceinline__ const uint32_t& at(uint32_t d) const {
+    return ring[((((d >> 2) % LR_BLOCKS) * 4u) + (3u - (d & 3u))) * LANE_NT];
   }
-  __device__ __forceinline__ void shift() {
-    b0 = b1;
-    b1 = b2;
-    b2 = b3;
-    b3 = b4;
-    b4 = b5;
-    b5 = b6;
-    b6 = b7;
-    --nblk;
+  __device__ __forceinline__ void put(const uint4& v) {
+    uint32_t* r = &ring[(((wr >> 2) % LR_BLOCKS) * 4u) * LANE_NT];
+    r[0] = v.x;
+    r[LANE_NT] = v.y;
+    r[2 * LANE_NT] = v.z;
+    r[3 * LANE_NT] = v.w;
+    wr += 4;
+  }
+  // issued once per symbol, by every lane: the word a refill would take next
+  // (read again unchanged when nothing was taken); used one symbol later
+  __device__ __forceinline__ void prefetch() {
+    pre = at(rd);
+    pre_ok = rd < wr;
   }
   __device__ __forceinline__ void init(const uint8_t* s, uint32_t E) {
     base = reinterpret_cast<const uint4*>(s);
-    q = (int32_t)((E - 1) >> 4);
-    refill();
-    const uint32_t j = ((E - 1) >> 2) & 3u;    // word of byte E-1 in its block
-    const uint32_t skip = 3u - ((E - 1) & 3u); // bytes of that word beyond E
-    cur = pick(b0, j) << (8u * skip);
+    bytes = s;
+    const int32_t q0 = (int32_t)((E - 1) >> 4);
+    wr = 0;
+    for (uint32_t k = 0; k < LR_BLOCKS; ++k) put(blk(q0 - (int32_t)k));
+    q = q0 - (int32_t)LR_BLOCKS;
+    // the first interval's blocks are requested now, so that every
+    // checkpoint finds its blocks requested one interval earlier
+    s0 = blk(q);
+    s1 = blk(q - 1);
+    q -= 2;
+    staged = true;
+    direct = false;
+    const uint32_t j = ((E - 1) >> 2) & 3u;     // word of byte E-1 in its block
+    const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of that word beyond E
+    rd = 3u - j;
+    cur = at(rd++) << (8u * skip);
     nb = 4u - skip;
-    w2 = j >= 1 ? pick(b0, j - 1) : 0u;
-    w1 = j >= 2 ? pick(b0, j - 2) : 0u;
-    w0 = j >= 3 ? b0.x : 0u;
-    nw = j;
-    shift();
+    prefetch();
   }
-  __device__ __forceinline__ uint32_t byte() {
-    if (nb == 0) {
-      if (nw == 0) {
-        if (nblk == 0) refill();
-        cur = b0.w;
-        w2 = b0.z;
-        w1 = b0.y;
-        w0 = b0.x;
-        nw = 3;
-        shift();
-      } else {
-        cur = w2;
-        w2 = w1;
-        w1 = w0;
-        --nw;
-      }
-      nb = 4;
+  // every lane of the wave, the same symbol: stage the blocks requested one
+  // interval ago, request the next ones
+  __device__ __forceinline__ void checkpoint() {
+    if (staged && wr - rd + 8u <= LR_WORDS) {
+      put(s0);
+      put(s1);
+      staged = false;
     }
+    if (!staged) {
+      s0 = blk(q);
+      s1 = blk(q - 1);
+      q -= 2;
+      staged = true;
+    }
+  }
+  // next stream byte; pos = bytes not yet consumed, this one included. At
+  // most one word is taken per symbol from a valid stream (2 bytes at most).
+  __device__ __forceinline__ uint32_t byte(uint32_t pos) {
+    if (nb == 0 && !direct) {
+      if (pre_ok) {
+        cur = pre;
+        nb = 4;
+        ++rd;
+        pre_ok = false;
+      } else {
+        direct = true;
+      }
+    }
+    if (direct) return bytes[pos - 1];
     const uint32_t r = cur >> 24;
     cur <<= 8;
     --nb;
@@ -393,8 +422,9 @@ struct LaneReader {
   }
 };
 
-__global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
+__global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   __shared__ uint32_t table[FCS_RANS_M];
+  __shared__ uint32_t rings[LR_WORDS * LANE_NT];
   for (uint32_t x = threadIdx.x; x < FCS_RANS_M; x += blockDim.x) {
     uint32_t s = 0;
     while (s + 1 < 16 && x >= a.cum[s + 1]) ++s;
@@ -411,29 +441,64 @@ __global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
     return;
   }
   uint32_t* nibw = a.nib + B.nib_off;
-  LaneReader rd;
+  LaneStream rd;
+  rd.ring = rings + threadIdx.x;
   rd.init(a.stream + B.str_off, E);
   uint32_t pos = E;
   uint32_t state = 0;
   for (int k = 0; k < 4; ++k) {
-    state = (state << 8) | rd.byte();
+    state = (state << 8) | rd.byte(pos);
     --pos;
   }
+  rd.prefetch();
   uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, cw = 0;
   for (uint32_t g = 0; g < nsym; ++g) {
+    if (g % LR_CHECK == 0 && g != 0) {
+      rd.checkpoint();
+      // the 32 symbols before g, as 4 nibble words
+      if (g % 32u == 0) reinterpret_cast<uint4*>(nibw)[(g >> 5) - 1] = make_uint4(w0, w1, w2, w3);
+    }
     const uint32_t x = state & (FCS_RANS_M - 1);
     const uint32_t e = table[x];
     state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
-    while (state < FCS_RANS_L && pos > 0) {
-      state = (state << 8) | rd.byte();
-      --pos;
+    // renormalise: the two bytes a valid stream can need come from registers;
+    // anything beyond (a corrupt stream, or a ring run dry) continues with
+    // direct loads in a separate loop, so that the common path never waits
+    // for global memory
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (state < FCS_RANS_L && pos > 0 && !rd.direct) {
+        if (rd.nb == 0) {
+          if (rd.pre_ok) {
+            rd.cur = rd.pre;
+            rd.nb = 4;
+            ++rd.rd;
+            rd.pre_ok = false;
+          } else {
+            rd.direct = true;
+          }
+        }
+        if (!rd.direct) {
+          state = (state << 8) | (rd.cur >> 24);
+          rd.cur <<= 8;
+          --rd.nb;
+          --pos;
+        }
+      }
     }
+    if (state < FCS_RANS_L && pos > 0) {
+      rd.direct = true;
+      do {
+        state = (state << 8) | rd.bytes[pos - 1];
+        --pos;
+      } while (state < FCS_RANS_L && pos > 0);
+    }
+    rd.prefetch();
     cw |= (e >> 25) << ((g & 7u) * 4u);
     if ((g & 7u) == 7u) {
       const uint32_t wi = (g >> 3) & 3u;
       if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
       cw = 0;
-      if (wi == 3) reinterpret_cast<uint4*>(nibw)[g >> 5] = make_uint4(w0, w1, w2, w3);
     }
   }
   if (nsym & 31u) {
@@ -442,6 +507,8 @@ __global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
       if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
     }
     reinterpret_cast<uint4*>(nibw)[nsym >> 5] = make_uint4(w0, w1, w2, w3);
+  } else {
+    reinterpret_cast<uint4*>(nibw)[(nsym >> 5) - 1] = make_uint4(w0, w1, w2, w3);
   }
   uint32_t status = 0;
   if (nsym & 1u) {
@@ -449,7 +516,7 @@ __global__ __launch_bounds__(256) void rans_lane_kernel(RansArgs a) {
     const uint32_t e = table[x];
     state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
     while (state < FCS_RANS_L && pos > 0) {
-      state = (state << 8) | rd.byte();
+      state = (state << 8) | rd.byte(pos);
       --pos;
     }
     if ((e >> 25) != 0) status |= FCS_ST_RANS_STATE;
