@@ -467,24 +467,21 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     // for global memory
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      if (state < FCS_RANS_L && pos > 0 && !rd.direct) {
-        if (rd.nb == 0) {
-          if (rd.pre_ok) {
-            rd.cur = rd.pre;
-            rd.nb = 4;
-            ++rd.rd;
-            rd.pre_ok = false;
-          } else {
-            rd.direct = true;
-          }
-        }
-        if (!rd.direct) {
-          state = (state << 8) | (rd.cur >> 24);
-          rd.cur <<= 8;
-          --rd.nb;
-          --pos;
-        }
-      }
+      // straight-line selects: a byte is taken when the state needs one and
+      // the lane has one (cur, or the prefetched word when cur is empty)
+      const bool need = state < FCS_RANS_L && pos > 0 && !rd.direct;
+      const bool refill = need && rd.nb == 0;
+      const bool have = !refill || rd.pre_ok;
+      rd.direct = rd.direct || !have;
+      const bool take = need && have;
+      const bool swap = refill && rd.pre_ok;
+      const uint32_t c = swap ? rd.pre : rd.cur;
+      rd.rd += swap ? 1u : 0u;
+      rd.pre_ok = rd.pre_ok && !swap;
+      state = take ? ((state << 8) | (c >> 24)) : state;
+      rd.cur = take ? (c << 8) : c;
+      rd.nb = take ? (swap ? 3u : rd.nb - 1u) : rd.nb;
+      pos -= take ? 1u : 0u;
     }
     if (state < FCS_RANS_L && pos > 0) {
       rd.direct = true;
