@@ -428,6 +428,23 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
       constexpr uint32_t REFB = (MODE == MODE_GLOBAL) ? FCS_REF32 : FCS_REF16;
       uint16_t* dum16 = reinterpret_cast<uint16_t*>(S.scr + SCR_DUM16) + lane;
       uint32_t* dum32 = S.scr + SCR_DUM32 + lane;
+      // an active parent owns 8 consecutive ops, so the lane's (at most 8)
+      // ops belong to at most two parents: read both once (clamped to the
+      // level's last parent; lanes past the level read a valid one and ignore it)
+      const uint32_t qlast = (K - 1) >> 3, q0 = min(k0 >> 3, qlast), q1 = min(q0 + 1, qlast);
+      uint32_t m0, pv0, m1, pv1;
+      if constexpr (C::PACKED) {
+        const uint32_t e0 = S.act32[q0], e1 = S.act32[q1];
+        m0 = e0 & 0xFFFFu;
+        pv0 = e0 >> 16;
+        m1 = e1 & 0xFFFFu;
+        pv1 = e1 >> 16;
+      } else {
+        m0 = S.act16[q0];
+        m1 = S.act16[q1];
+        pv0 = io.vcur[m0];
+        pv1 = io.vcur[m1];
+      }
 #pragma unroll 2
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t g = g0 + j;
@@ -435,17 +452,9 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         const uint32_t op = v >> 1;
         const bool isop = (iso >> j) & 1u;
         const bool live = isop && k < K;
-        const uint32_t kk = live ? k : 0u;
-        uint32_t m, pv;
-        if constexpr (C::PACKED) {
-          const uint32_t e = S.act32[kk >> 3];
-          m = e & 0xFFFFu;
-          pv = e >> 16;
-        } else {
-          m = S.act16[kk >> 3];
-          pv = io.vcur[m];
-        }
-        const uint32_t c = kk & 7u, n = (m << 3) | c;
+        const bool second = (k >> 3) != q0;
+        const uint32_t m = second ? m1 : m0, pv = second ? pv1 : pv0;
+        const uint32_t c = k & 7u, n = (m << 3) | c;
         // Rx/Ry/Rz: neighbour along the axis by dilated-integer arithmetic on
         // the Morton code (x-1 for an even coordinate, x+1 for an odd one)
         const uint32_t ax = (op >= FCS_OP_RX && op <= FCS_OP_RZ) ? op - 1 : 0u;
