@@ -284,7 +284,9 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
     state = f * (state >> FCS_RANS_LOGM) + r;
     while (state < FCS_RANS_L && pos > 0) {
       if (avail < 8u) refill();
-      state = (state << 8) | (uint32_t)(win >> 56);
+      uint32_t byte = (uint32_t)(win >> 56);
+      __asm__ volatile("" : "+s"(byte));  // scalar: no vector funnel shift on the chain
+      state = (state << 8) | byte;
       win <<= 8;
       avail -= 8u;
       --pos;
