@@ -861,6 +861,7 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
       // autotuner: every mapping over the whole volume (identical outputs),
       // after a one-brick launch of each so code loading is not timed
       RansArgs one = a;
+      one.list = v->d_rlist + (v->nb - 1);  // the shortest stream (the list runs longest first)
       one.nlist = 1;
       constexpr int kModes = 6;
       const int modes[kModes] = {FCS_RANS_WAVE, FCS_RANS_WAVE_BALLOT, FCS_RANS_WAVE_LDS, FCS_RANS_LANE,
