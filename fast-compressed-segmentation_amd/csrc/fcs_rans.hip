@@ -349,7 +349,8 @@ struct LaneStream {
   uint32_t* ring;     // LDS, LR_WORDS rows of LANE_NT words, this lane's column
   int32_t q;          // next block to request (decreasing; below 0 reads as zeros)
   uint32_t rd, wr;    // ring words: next unread / written so far (consumption order)
-  uint32_t cur, nb;   // current word (next byte in bits 31..24), bytes left in it
+  uint64_t bb;        // upcoming stream bytes, the next one in bits 63..56
+  uint32_t nbits;     // valid bits in bb (a multiple of 8)
   uint32_t pre;       // ring word rd, read at the end of the previous symbol
   bool pre_ok, staged, direct;
   uint4 s0, s1;       // blocks requested at the last checkpoint
@@ -375,6 +376,18 @@ struct LaneStream {
     pre = at(rd);
     pre_ok = rd < wr;
   }
+  // fewer than 2 bytes left: append the prefetched word below them. A ring
+  // run dry (a corrupt stream only) switches the lane to direct byte loads.
+  __device__ __forceinline__ void refill() {
+    if (pre_ok) {
+      bb |= (uint64_t)pre << (32u - nbits);
+      nbits += 32u;
+      ++rd;
+      pre_ok = false;
+    } else {
+      direct = true;
+    }
+  }
   __device__ __forceinline__ void init(const uint8_t* s, uint32_t E) {
     base = reinterpret_cast<const uint4*>(s);
     bytes = s;
@@ -392,8 +405,10 @@ struct LaneStream {
     const uint32_t j = ((E - 1) >> 2) & 3u;     // word of byte E-1 in its block
     const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of that word beyond E
     rd = 3u - j;
-    cur = at(rd++) << (8u * skip);
-    nb = 4u - skip;
+    bb = (uint64_t)(at(rd++) << (8u * skip)) << 32;
+    nbits = 32u - 8u * skip;
+    bb |= (uint64_t)at(rd++) << (32u - nbits);
+    nbits += 32u;  // 40..64 bits: the initial state and more
     prefetch();
   }
   // every lane of the wave, the same symbol: stage the blocks requested one
@@ -411,24 +426,27 @@ struct LaneStream {
       staged = true;
     }
   }
-  // next stream byte; pos = bytes not yet consumed, this one included. At
-  // most one word is taken per symbol from a valid stream (2 bytes at most).
-  __device__ __forceinline__ uint32_t byte(uint32_t pos) {
-    if (nb == 0 && !direct) {
-      if (pre_ok) {
-        cur = pre;
-        nb = 4;
-        ++rd;
-        pre_ok = false;
-      } else {
-        direct = true;
+  // renormalise after a symbol: while state < 2^23 and bytes are left, shift
+  // the next byte in. From a state of at least 2^7 that is n = [s < 2^23] +
+  // [s < 2^15] <= 2 bytes, taken from bb at once; a smaller state (a corrupt
+  // stream) continues with direct loads, byte by byte, as the sequential rule.
+  __device__ __forceinline__ void renorm(uint32_t& state, uint32_t& pos) {
+    if (!direct && state < 128u && pos > 2u) direct = true;
+    if (!direct) {
+      const uint32_t n = min((state < FCS_RANS_L ? 1u : 0u) + (state < (1u << 15) ? 1u : 0u), pos);
+      const uint32_t sh = 8u * n;
+      const uint64_t top = n ? (bb >> (64u - sh)) : 0ull;
+      state = (uint32_t)(((uint64_t)state << sh) | top);
+      bb <<= sh;
+      nbits -= sh;
+      pos -= n;
+      if (nbits < 16u) refill();
+    } else {
+      while (state < FCS_RANS_L && pos > 0) {
+        state = (state << 8) | bytes[pos - 1];
+        --pos;
       }
     }
-    if (direct) return bytes[pos - 1];
-    const uint32_t r = cur >> 24;
-    cur <<= 8;
-    --nb;
-    return r;
   }
 };
 
@@ -454,13 +472,12 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   LaneStream rd;
   rd.ring = rings + threadIdx.x;
   rd.init(a.stream + B.str_off, E);
-  uint32_t pos = E;
-  uint32_t state = 0;
-  for (int k = 0; k < 4; ++k) {
-    state = (state << 8) | rd.byte(pos);
-    --pos;
-  }
-  rd.prefetch();
+  // the initial state: the stream's last 4 bytes
+  uint32_t pos = E - 4;
+  uint32_t state = (uint32_t)(rd.bb >> 32);
+  rd.bb <<= 32;
+  rd.nbits -= 32u;
+  if (rd.nbits < 16u) rd.refill();
   uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, cw = 0;
   for (uint32_t g = 0; g < nsym; ++g) {
     if (g % LR_CHECK == 0 && g != 0) {
@@ -471,35 +488,7 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     const uint32_t x = state & (FCS_RANS_M - 1);
     const uint32_t e = table[x];
     state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
-    // renormalise: the two bytes a valid stream can need come from registers;
-    // anything beyond (a corrupt stream, or a ring run dry) continues with
-    // direct loads in a separate loop, so that the common path never waits
-    // for global memory
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      // straight-line selects: a byte is taken when the state needs one and
-      // the lane has one (cur, or the prefetched word when cur is empty)
-      const bool need = state < FCS_RANS_L && pos > 0 && !rd.direct;
-      const bool refill = need && rd.nb == 0;
-      const bool have = !refill || rd.pre_ok;
-      rd.direct = rd.direct || !have;
-      const bool take = need && have;
-      const bool swap = refill && rd.pre_ok;
-      const uint32_t c = swap ? rd.pre : rd.cur;
-      rd.rd += swap ? 1u : 0u;
-      rd.pre_ok = rd.pre_ok && !swap;
-      state = take ? ((state << 8) | (c >> 24)) : state;
-      rd.cur = take ? (c << 8) : c;
-      rd.nb = take ? (swap ? 3u : rd.nb - 1u) : rd.nb;
-      pos -= take ? 1u : 0u;
-    }
-    if (state < FCS_RANS_L && pos > 0) {
-      rd.direct = true;
-      do {
-        state = (state << 8) | rd.bytes[pos - 1];
-        --pos;
-      } while (state < FCS_RANS_L && pos > 0);
-    }
+    rd.renorm(state, pos);
     rd.prefetch();
     cw |= (e >> 25) << ((g & 7u) * 4u);
     if ((g & 7u) == 7u) {
@@ -522,10 +511,7 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     const uint32_t x = state & (FCS_RANS_M - 1);
     const uint32_t e = table[x];
     state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
-    while (state < FCS_RANS_L && pos > 0) {
-      state = (state << 8) | rd.byte(pos);
-      --pos;
-    }
+    rd.renorm(state, pos);
     if ((e >> 25) != 0) status |= FCS_ST_RANS_STATE;
   }
   if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
