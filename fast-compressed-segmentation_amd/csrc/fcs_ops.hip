@@ -23,6 +23,11 @@
 // 8 consecutive voxels (two or four 16-byte stores). Children of stopped
 // parents are written straight from the parent's value. A level whose nodes
 // are all stopped ends the brick: the rest is a direct expansion.
+//
+// Large launches run in two kernels: ops_head_kernel decodes the non-final
+// levels up to 3 of every request, one wave per request, and leaves the
+// level state in the request's output; ops_kernel reads it back and decodes
+// the remaining levels with the whole workgroup.
 #include <hip/hip_runtime.h>
 
 #include "fcs_common.h"

@@ -15,9 +15,11 @@
 //     nibble word k of a 64-word group is kept in lane k, stored 256 B at a time.
 //     Lowest latency per brick: for volumes with few, long bricks.
 //     The decode table is read through the scalar cache, or copied into LDS.
-//   rans_lane_kernel: one lane per brick, 64 bricks per wave, symbol table in
-//     LDS, stream fetched 128 bytes per lane at a time. Highest throughput:
-//     for volumes with many bricks.
+//   rans_win_kernel: one wave per brick, the byte source in scalar registers.
+//   rans_lane_kernel: one lane per brick, 64 bricks per wave (launched longest
+//     first), symbol table in LDS, each lane's stream through a ring in LDS
+//     refilled at wave-wide checkpoints. Highest throughput: for volumes with
+//     many bricks.
 #include <hip/hip_runtime.h>
 
 #include "fcs_common.h"
