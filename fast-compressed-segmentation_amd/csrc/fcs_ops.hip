@@ -917,10 +917,6 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
   // level 0 state: the root (palette index 0) is the one active node
   LevelVars<LT> lv{S.sA, S.st0, S.st1, 1u, 1u, 0u, 0u};
   LevelConst<LT> lc{nibw, nsym, w_end, t, L, P, pal, lp, o, pyr, staged};
-  // Non-final levels whose children fit one wave's tiles (8, 64, 512) are
-  // decoded by the first wave alone, with wave-level syncs; the other waves
-  // wait at the handoff barrier. A level whose children are all stopped ends
-  // the brick there; its expansion then runs with the whole workgroup.
   uint32_t l = 0, ended_at = 0;
   if (use_head) {
     // the head pass decoded the first levels: its state, held by the
@@ -960,6 +956,10 @@ __device__ void decode_request(const OpsArgs<LT>& a, const Lds<LT>& S, uint32_t 
     }
     lds_barrier();
   } else {
+    // Non-final levels whose children fit one wave's tiles (8, 64, 512) are
+    // decoded by the first wave alone, with wave-level syncs; the other waves
+    // wait at the handoff barrier. A level whose children are all stopped ends
+    // the brick there; its expansion then runs with the whole workgroup.
     if (wave == 0) {
       Blocks sb;
       sb.bc = 0;
