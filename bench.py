@@ -332,6 +332,28 @@ def bench_config3(args):
                     "gvox_s": bvox / (batch_ms * 1e-3) / 1e9, "gbs": bbytes / (batch_ms * 1e-3) / 1e9,
                     "status_ok": bool(np.all(bst == 0))},
     }
+    if not args.no_cpu_baseline:
+        # the oracle restatement (rANS + finest-LOD decode per brick) on every
+        # host core, over random bricks of the same volume for a bounded time
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as O
+
+        threads = os.cpu_count() or 1
+        ds = O.Dataset(data)
+        sample = np.random.default_rng(5).permutation(nb)
+        done = 0
+        t2 = time.time()
+        with ThreadPoolExecutor(threads) as ex:
+            while done < nb and time.time() - t2 < args.cpu_seconds * 0.6:
+                chunk = sample[done:done + 16 * threads]
+                list(ex.map(lambda i: ds.decode(int(i), L), chunk))
+                done += len(chunk)
+        cpu_s = time.time() - t2
+        out["cpu_baseline"] = {
+            "value": done * (1 << (3 * L)) / cpu_s / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "port",
+            "sample": f"oracle rANS + finest-LOD decode of {done} random bricks of the volume on {threads} threads",
+        }
     print(json.dumps(out))
 
 
