@@ -328,29 +328,35 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
 
 // ---------------------------------------------------------- lane per brick
 // Each lane decodes one brick. Its stream is read backwards through a ring of
-// LR_BLOCKS 16-byte blocks in LDS and two words in registers (cur, the next
-// bytes; nxt, the word after it, read from the ring ahead of use). Global
-// memory is touched only every LR_CHECK symbols, an iteration all lanes of the
-// wave share: the blocks loaded at the previous checkpoint go into the ring,
-// the next ones are requested, and finished nibble words are stored. Waits for
-// global memory (which complete in issue order, per wave) therefore only ever
-// cover operations issued a whole interval earlier; a lane refilling its
-// words at its own moment waits for LDS only. A valid stream consumes at most
-// 2 bytes per symbol (the state never falls below 2^11 before renormalising),
-// so a ring holding two intervals never runs dry; a corrupt one that does
+// LR_BLOCKS 16-byte blocks in LDS (kept in consumption order: ring row d % 16
+// holds the d-th word read) and a 64-bit bit buffer in registers, topped up
+// from a ring word read one symbol ahead of use. Global memory is touched only
+// every LR_CHECK symbols, an iteration all lanes of the wave share: the blocks
+// loaded at the previous checkpoint go into the ring, the next ones are
+// requested, and finished nibble words are stored. Waits for global memory
+// (which complete in issue order, per wave) therefore only ever cover
+// operations issued a whole interval earlier; a lane refilling its buffer at
+// its own moment waits for LDS only. A valid stream consumes at most 2 bytes
+// per symbol (the state never falls below 2^11 before renormalising), so a
+// ring holding two intervals never runs dry; a corrupt one that does
 // continues with direct byte loads, so every stream decodes exactly as the
 // sequential rule reads it.
 constexpr uint32_t LANE_NT = 256;
 constexpr uint32_t LR_BLOCKS = 4;             // 64 bytes per lane
 constexpr uint32_t LR_WORDS = LR_BLOCKS * 4;
-constexpr uint32_t LR_CHECK = 16;                  // symbols between checkpoints (<= 32 bytes)
+constexpr uint32_t LR_CHECK = 16;             // symbols between checkpoints (<= 32 bytes)
+constexpr uint32_t LR_ROW = LANE_NT * 4u;     // bytes between ring rows
+constexpr uint32_t LR_MASK = (LR_WORDS - 1u) * LR_ROW;
 
 struct LaneStream {
   const uint4* base;  // the brick's stream as 16-byte blocks
   const uint8_t* bytes;
-  uint32_t* ring;     // LDS, LR_WORDS rows of LANE_NT words, this lane's column
+  char* ring;         // LDS, LR_WORDS rows of LANE_NT words, this lane's column
   int32_t q;          // next block to request (decreasing; below 0 reads as zeros)
-  uint32_t rd, wr;    // ring words: next unread / written so far (consumption order)
+  // ring words read / written so far, times LR_ROW (so that the low bits
+  // masked by LR_MASK are the row's byte offset); rd never passes wr, and
+  // they differ by at most LR_WORDS words, so comparisons survive wrapping
+  uint32_t rdx, wrx;
   uint64_t bb;        // upcoming stream bytes, the next one in bits 63..56
   uint32_t nbits;     // valid bits in bb (a multiple of 8)
   uint32_t pre;       // ring word rd, read at the end of the previous symbol
@@ -360,23 +366,23 @@ struct LaneStream {
   __device__ __forceinline__ uint4 blk(int32_t k) const {
     return k >= 0 ? base[k] : make_uint4(0u, 0u, 0u, 0u);
   }
-  // consumption word d: block (first - d / 4), word 3 - d % 4 of it
-  __device__ __forceinline__ const uint32_t& at(uint32_t d) const {
-    return ring[((((d >> 2) % LR_BLOCKS) * 4u) + (3u - (d & 3u))) * LANE_NT];
+  __device__ __forceinline__ uint32_t at(uint32_t dx) const {
+    return *reinterpret_cast<const uint32_t*>(ring + (dx & LR_MASK));
   }
+  // a block's words are read last to first
   __device__ __forceinline__ void put(const uint4& v) {
-    uint32_t* r = &ring[(((wr >> 2) % LR_BLOCKS) * 4u) * LANE_NT];
-    r[0] = v.x;
-    r[LANE_NT] = v.y;
-    r[2 * LANE_NT] = v.z;
-    r[3 * LANE_NT] = v.w;
-    wr += 4;
+    char* r = ring + (wrx & LR_MASK);
+    *reinterpret_cast<uint32_t*>(r) = v.w;
+    *reinterpret_cast<uint32_t*>(r + LR_ROW) = v.z;
+    *reinterpret_cast<uint32_t*>(r + 2u * LR_ROW) = v.y;
+    *reinterpret_cast<uint32_t*>(r + 3u * LR_ROW) = v.x;
+    wrx += 4u * LR_ROW;
   }
   // issued once per symbol, by every lane: the word a refill would take next
   // (read again unchanged when nothing was taken); used one symbol later
   __device__ __forceinline__ void prefetch() {
-    pre = at(rd);
-    pre_ok = rd < wr;
+    pre = at(rdx);
+    pre_ok = rdx != wrx;
   }
   // fewer than 2 bytes left: append the prefetched word below them. A ring
   // run dry (a corrupt stream only) switches the lane to direct byte loads.
@@ -384,7 +390,7 @@ struct LaneStream {
     if (pre_ok) {
       bb |= (uint64_t)pre << (32u - nbits);
       nbits += 32u;
-      ++rd;
+      rdx += LR_ROW;
       pre_ok = false;
     } else {
       direct = true;
@@ -394,7 +400,7 @@ struct LaneStream {
     base = reinterpret_cast<const uint4*>(s);
     bytes = s;
     const int32_t q0 = (int32_t)((E - 1) >> 4);
-    wr = 0;
+    wrx = 0;
     for (uint32_t k = 0; k < LR_BLOCKS; ++k) put(blk(q0 - (int32_t)k));
     q = q0 - (int32_t)LR_BLOCKS;
     // the first interval's blocks are requested now, so that every
@@ -406,17 +412,19 @@ struct LaneStream {
     direct = false;
     const uint32_t j = ((E - 1) >> 2) & 3u;     // word of byte E-1 in its block
     const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of that word beyond E
-    rd = 3u - j;
-    bb = (uint64_t)(at(rd++) << (8u * skip)) << 32;
+    rdx = (3u - j) * LR_ROW;
+    bb = (uint64_t)(at(rdx) << (8u * skip)) << 32;
+    rdx += LR_ROW;
     nbits = 32u - 8u * skip;
-    bb |= (uint64_t)at(rd++) << (32u - nbits);
+    bb |= (uint64_t)at(rdx) << (32u - nbits);
+    rdx += LR_ROW;
     nbits += 32u;  // 40..64 bits: the initial state and more
     prefetch();
   }
   // every lane of the wave, the same symbol: stage the blocks requested one
   // interval ago, request the next ones
   __device__ __forceinline__ void checkpoint() {
-    if (staged && wr - rd + 8u <= LR_WORDS) {
+    if (staged && wrx - rdx + 8u * LR_ROW <= LR_WORDS * LR_ROW) {
       put(s0);
       put(s1);
       staged = false;
@@ -432,18 +440,27 @@ struct LaneStream {
   // the next byte in. From a state of at least 2^7 that is n = [s < 2^23] +
   // [s < 2^15] <= 2 bytes, taken from bb at once; a smaller state (a corrupt
   // stream) continues with direct loads, byte by byte, as the sequential rule.
+  // Straight-line code but for that rare loop: a lane in direct mode takes
+  // n = 0 and no refill.
   __device__ __forceinline__ void renorm(uint32_t& state, uint32_t& pos) {
-    if (!direct && state < 128u && pos > 2u) direct = true;
-    if (!direct) {
-      const uint32_t n = min((state < FCS_RANS_L ? 1u : 0u) + (state < (1u << 15) ? 1u : 0u), pos);
-      const uint32_t sh = 8u * n;
-      const uint64_t top = n ? (bb >> (64u - sh)) : 0ull;
-      state = (uint32_t)(((uint64_t)state << sh) | top);
-      bb <<= sh;
-      nbits -= sh;
-      pos -= n;
-      if (nbits < 16u) refill();
-    } else {
+    direct = direct || state < 128u;
+    const uint32_t nn = min((state < FCS_RANS_L ? 1u : 0u) + (state < (1u << 15) ? 1u : 0u), pos);
+    const uint32_t n = direct ? 0u : nn;
+    const uint32_t sh = 8u * n;
+    // (state : top word of bb) >> (32 - sh): the state shifted up by sh bits
+    // with the next n bytes below
+    state = (uint32_t)((((uint64_t)state << 32) | (bb >> 32)) >> (32u - sh));
+    bb <<= sh;
+    nbits -= sh;
+    pos -= n;
+    const bool low = !direct && nbits < 16u;
+    const bool ok = low && pre_ok;
+    const uint32_t pw = ok ? pre : 0u;
+    bb |= (uint64_t)pw << ((32u - nbits) & 63u);
+    nbits += ok ? 32u : 0u;
+    rdx += ok ? LR_ROW : 0u;
+    direct = direct || (low && !ok);  // a ring run dry: a corrupt stream
+    if (direct) {
       while (state < FCS_RANS_L && pos > 0) {
         state = (state << 8) | bytes[pos - 1];
         --pos;
@@ -452,13 +469,21 @@ struct LaneStream {
   }
 };
 
+// decode table entry for x: symbol (bits 0..3), x - cum[symbol] (bits 4..15),
+// freq[symbol] (bits 16..28)
+__device__ __forceinline__ uint32_t lane_step(uint32_t& state, const uint32_t* table) {
+  const uint32_t e = table[state & (FCS_RANS_M - 1)];
+  state = (e >> 16) * (state >> FCS_RANS_LOGM) + ((e >> 4) & 0xFFFu);
+  return e;
+}
+
 __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   __shared__ uint32_t table[FCS_RANS_M];
   __shared__ uint32_t rings[LR_WORDS * LANE_NT];
   for (uint32_t x = threadIdx.x; x < FCS_RANS_M; x += blockDim.x) {
     uint32_t s = 0;
     while (s + 1 < 16 && x >= a.cum[s + 1]) ++s;
-    table[x] = (s << 25) | (a.freq[s] << 12) | a.cum[s];
+    table[x] = s | ((x - a.cum[s]) << 4) | (a.freq[s] << 16);
   }
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -472,7 +497,7 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   }
   uint32_t* nibw = a.nib + B.nib_off;
   LaneStream rd;
-  rd.ring = rings + threadIdx.x;
+  rd.ring = reinterpret_cast<char*>(rings + threadIdx.x);
   rd.init(a.stream + B.str_off, E);
   // the initial state: the stream's last 4 bytes
   uint32_t pos = E - 4;
@@ -480,41 +505,59 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   rd.bb <<= 32;
   rd.nbits -= 32u;
   if (rd.nbits < 16u) rd.refill();
-  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, cw = 0;
-  for (uint32_t g = 0; g < nsym; ++g) {
-    if (g % LR_CHECK == 0 && g != 0) {
-      rd.checkpoint();
-      // the 32 symbols before g, as 4 nibble words
-      if (g % 32u == 0) reinterpret_cast<uint4*>(nibw)[(g >> 5) - 1] = make_uint4(w0, w1, w2, w3);
-    }
-    const uint32_t x = state & (FCS_RANS_M - 1);
-    const uint32_t e = table[x];
-    state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
+  auto step = [&]() -> uint32_t {
+    const uint32_t e = lane_step(state, table);
     rd.renorm(state, pos);
     rd.prefetch();
-    cw |= (e >> 25) << ((g & 7u) * 4u);
-    if ((g & 7u) == 7u) {
-      const uint32_t wi = (g >> 3) & 3u;
-      if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
-      cw = 0;
+    return e;
+  };
+  // 8 symbols -> one nibble word (each symbol shifted in at the top, so the
+  // first ends in bits 0..3); 4 words -> one 16-byte store. The word counter
+  // is the same for every active lane: checkpoints (every 16 symbols) and
+  // stores happen at iterations the whole wave shares.
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+  const uint32_t nfull = nsym >> 3;
+  for (uint32_t wd = 0; wd < nfull; ++wd) {
+    if ((wd & 1u) == 0 && wd != 0) {
+      rd.checkpoint();
+      // the 32 symbols before, as 4 nibble words
+      if ((wd & 3u) == 0) reinterpret_cast<uint4*>(nibw)[(wd >> 2) - 1] = make_uint4(w0, w1, w2, w3);
     }
+    uint32_t word = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) word = (word >> 4) | (step() << 28);
+    const uint32_t wi = wd & 3u;
+    w0 = wi == 0 ? word : w0;
+    w1 = wi == 1 ? word : w1;
+    w2 = wi == 2 ? word : w2;
+    w3 = wi == 3 ? word : w3;
+  }
+  const uint32_t r = nsym & 7u;
+  if (r) {
+    if ((nfull & 1u) == 0 && nfull != 0) {
+      rd.checkpoint();
+      // the partial word opens a new group: the previous one is complete
+      if ((nfull & 3u) == 0) reinterpret_cast<uint4*>(nibw)[(nfull >> 2) - 1] = make_uint4(w0, w1, w2, w3);
+    }
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < r; ++k) word = (word >> 4) | (step() << 28);
+    word >>= 32u - 4u * r;
+    const uint32_t wi = nfull & 3u;
+    w0 = wi == 0 ? word : w0;
+    w1 = wi == 1 ? word : w1;
+    w2 = wi == 2 ? word : w2;
+    w3 = wi == 3 ? word : w3;
   }
   if (nsym & 31u) {
-    const uint32_t wi = (nsym >> 3) & 3u;
-    if (nsym & 7u) {
-      if (wi == 0) w0 = cw; else if (wi == 1) w1 = cw; else if (wi == 2) w2 = cw; else w3 = cw;
-    }
     reinterpret_cast<uint4*>(nibw)[nsym >> 5] = make_uint4(w0, w1, w2, w3);
   } else {
     reinterpret_cast<uint4*>(nibw)[(nsym >> 5) - 1] = make_uint4(w0, w1, w2, w3);
   }
   uint32_t status = 0;
   if (nsym & 1u) {
-    const uint32_t x = state & (FCS_RANS_M - 1);
-    const uint32_t e = table[x];
-    state = ((e >> 12) & 0x1FFFu) * (state >> FCS_RANS_LOGM) + (x - (e & 0xFFFu));
+    const uint32_t e = lane_step(state, table);
     rd.renorm(state, pos);
-    if ((e >> 25) != 0) status |= FCS_ST_RANS_STATE;
+    if ((e & 15u) != 0) status |= FCS_ST_RANS_STATE;
   }
   if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
   a.res[b] = RansResult{status, 0u};
