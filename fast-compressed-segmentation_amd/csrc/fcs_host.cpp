@@ -652,7 +652,8 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     d.P = raw[i].P;
     d.nib_off = nib_total;
     str_total += (raw[i].E + 15) & ~15ull;
-    nib_total += 4ull * std::max<uint64_t>(1, ((uint64_t)raw[i].nsym + 31) / 32);
+    // whole 64-byte lines (the lane decoder stores line by line)
+    nib_total += 16ull * std::max<uint64_t>(1, ((uint64_t)raw[i].nsym + 127) / 128);
     pal_total += raw[i].P;
     v->compressed += raw[i].E;
     v->symbols += raw[i].nsym;

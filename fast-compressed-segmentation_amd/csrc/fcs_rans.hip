@@ -512,47 +512,44 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     return e;
   };
   // 8 symbols -> one nibble word (each symbol shifted in at the top, so the
-  // first ends in bits 0..3); 4 words -> one 16-byte store. The word counter
-  // is the same for every active lane: checkpoints (every 16 symbols) and
-  // stores happen at iterations the whole wave shares.
-  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+  // first ends in bits 0..3); 16 words -> one 64-byte line, stored whole. The
+  // word counter is the same for every active lane (checkpoints every 16
+  // symbols and stores happen at iterations the whole wave shares), so the
+  // word array is indexed uniformly.
+  uint32_t w[16];
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) w[k] = 0;
+  auto flush = [&](uint32_t line) {
+    uint4* o = reinterpret_cast<uint4*>(nibw) + line * 4u;
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    o[2] = make_uint4(w[8], w[9], w[10], w[11]);
+    o[3] = make_uint4(w[12], w[13], w[14], w[15]);
+  };
   const uint32_t nfull = nsym >> 3;
   for (uint32_t wd = 0; wd < nfull; ++wd) {
     if ((wd & 1u) == 0 && wd != 0) {
       rd.checkpoint();
-      // the 32 symbols before, as 4 nibble words
-      if ((wd & 3u) == 0) reinterpret_cast<uint4*>(nibw)[(wd >> 2) - 1] = make_uint4(w0, w1, w2, w3);
+      if ((wd & 15u) == 0) flush((wd >> 4) - 1);
     }
     uint32_t word = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) word = (word >> 4) | (step() << 28);
-    const uint32_t wi = wd & 3u;
-    w0 = wi == 0 ? word : w0;
-    w1 = wi == 1 ? word : w1;
-    w2 = wi == 2 ? word : w2;
-    w3 = wi == 3 ? word : w3;
+    w[wd & 15u] = word;
   }
   const uint32_t r = nsym & 7u;
   if (r) {
     if ((nfull & 1u) == 0 && nfull != 0) {
       rd.checkpoint();
-      // the partial word opens a new group: the previous one is complete
-      if ((nfull & 3u) == 0) reinterpret_cast<uint4*>(nibw)[(nfull >> 2) - 1] = make_uint4(w0, w1, w2, w3);
+      // the partial word opens a new line: the previous one is complete
+      if ((nfull & 15u) == 0) flush((nfull >> 4) - 1);
     }
     uint32_t word = 0;
     for (uint32_t k = 0; k < r; ++k) word = (word >> 4) | (step() << 28);
-    word >>= 32u - 4u * r;
-    const uint32_t wi = nfull & 3u;
-    w0 = wi == 0 ? word : w0;
-    w1 = wi == 1 ? word : w1;
-    w2 = wi == 2 ? word : w2;
-    w3 = wi == 3 ? word : w3;
+    w[nfull & 15u] = word >> (32u - 4u * r);
   }
-  if (nsym & 31u) {
-    reinterpret_cast<uint4*>(nibw)[nsym >> 5] = make_uint4(w0, w1, w2, w3);
-  } else {
-    reinterpret_cast<uint4*>(nibw)[(nsym >> 5) - 1] = make_uint4(w0, w1, w2, w3);
-  }
+  // the last line (nibble regions are whole 64-byte lines)
+  flush((nsym - 1u) >> 7);
   uint32_t status = 0;
   if (nsym & 1u) {
     const uint32_t e = lane_step(state, table);
