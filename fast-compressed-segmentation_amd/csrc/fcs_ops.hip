@@ -427,7 +427,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
     const uint32_t my_ops = (myX ? v1 : v0) & 0xFFFFu;
     if (__ballot(my_ops != 0 && op_base + (ex & 0xFFFFu) < K) != 0) {
       const uint32_t k0 = op_base + (ex & 0xFFFFu), ip_start = io.ip0 + pa_base + (ex >> 16);
-      uint32_t k = k0, ipx = ip_start;
+      uint32_t k = k0;
       const uint32_t iso = par ? iso1 : iso0;
       // node references: bit 15 of the LDS u16 forms, bit 31 of the labels
       constexpr uint32_t REFB = (MODE == MODE_GLOBAL) ? FCS_REF32 : FCS_REF16;
@@ -450,9 +450,9 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         pv0 = io.vcur[m0];
         pv1 = io.vcur[m1];
       }
+      bool e_pal = false, e_ovr = false, e_op = false;
 #pragma unroll 2
       for (uint32_t j = 0; j < 8; ++j) {
-        const uint32_t g = g0 + j;
         const uint32_t v = (word >> (4 * j)) & 0xFu;
         const uint32_t op = v >> 1;
         const bool isop = (iso >> j) & 1u;
@@ -468,24 +468,19 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         const bool odd = (n >> ax) & 1u;
         const bool outside = odd ? (xa == mA) : (xa == 0u);
         const uint32_t nb = (odd ? (((xa | ~mA) + 1u) & mA) : ((xa - 1u) & mA)) | rest;
-        uint32_t nv = io.vcur[nb >> 3];
-        // keep the read unconditional: a branch around it costs more than the read
-        asm volatile("" : "+v"(nv));
+        const uint32_t nv = io.vcur[nb >> 3];
         // outside the brick the reference reads the parent's output slot,
-        // which child 0 has already overwritten when c > 0
-        const uint32_t rval = outside ? ((c == 0) ? pv : (REFB | (8 * m))) : ((nb < n) ? (REFB | nb) : nv);
-        // Pa / P0 / Pd: palette cursor (Pd: delta nibble dn selects ipx - dn - 1)
-        ipx += (live && op == FCS_OP_PA) ? 1u : 0u;
-        const bool isd = (op == FCS_OP_PD);
-        const uint32_t sub = isd ? (((j < 7) ? (word >> (4 * (j + 1))) : wnext) & 0xFu) + 1u : 0u;
-        const uint32_t pidx = ipx - sub;  // wraps past P when the delta reaches before the palette
-        const bool bad_pal = pidx >= P;
-        const uint32_t pval = min(pidx, P - 1);
-        uint32_t val = (op == FCS_OP_RP) ? pv : (op <= FCS_OP_RZ) ? rval : (op <= FCS_OP_PD) ? pval : 0u;
-        uint32_t sb = (op >= FCS_OP_PA && op <= FCS_OP_PD && bad_pal) ? FCS_ST_PALETTE_RANGE : 0u;
-        sb |= (isd && g + 1 >= ne) ? FCS_ST_STREAM_OVERRUN : 0u;
-        sb |= (op > FCS_OP_PD) ? FCS_ST_BAD_OP : 0u;
-        st |= live ? sb : 0u;
+        // which child 0 has already overwritten when c > 0. Bitwise selects
+        // (v_bfi): the read stays unconditional and the code straight-line.
+        const uint32_t mo = 0u - (uint32_t)outside, mc = 0u - (uint32_t)(c == 0), mb = 0u - (uint32_t)(nb < n);
+        const uint32_t oval = (mc & pv) | (~mc & (REFB | (8 * m)));
+        const uint32_t ival = (mb & (REFB | nb)) | (~mb & nv);
+        const uint32_t rval = (mo & oval) | (~mo & ival);
+        // palette ops (about 1 % of ops) are written below, in a loop over
+        // only those nibbles; here they take a placeholder
+        const uint32_t val = (op == FCS_OP_RP) ? pv : (op <= FCS_OP_RZ) ? rval : 0u;
+        // status as a lane mask (scalar ors), folded in once per word
+        e_op = e_op || (live && op > FCS_OP_PD);
         if constexpr (MODE == MODE_LEVEL) {
           *(live ? &io.vnext[n] : dum16) = (uint16_t)val;
           atomicOr(live ? &io.snext[n >> 5] : dum32, (live && (v & 1u)) ? (1u << (n & 31)) : 0u);
@@ -496,6 +491,44 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         }
         k += isop ? 1u : 0u;
       }
+      {
+        // Pa / P0 / Pd. Nibble masks with the flag in bit 4j: op nibbles,
+        // Pa ops (nibble 8|9), all palette ops (nibbles 8..13)
+        uint32_t isoN = iso & 0xFFu;
+        isoN = (isoN | (isoN << 12)) & 0x000F000Fu;
+        isoN = (isoN | (isoN << 6)) & 0x03030303u;
+        isoN = (isoN | (isoN << 3)) & 0x11111111u;
+        const uint32_t b3 = word >> 3, b2 = word >> 2, b1 = word >> 1;
+        const uint32_t paN = b3 & ~b2 & ~b1 & isoN;
+        uint32_t pm = b3 & ~(b2 & b1) & isoN;
+        while (pm != 0u) {
+          const uint32_t j4 = __builtin_ctz(pm);  // 4 * nibble
+          pm &= pm - 1u;
+          const uint32_t below = (1u << j4) - 1u;
+          const uint32_t kj = k0 + __popc(isoN & below);
+          if (kj >= K) break;  // past the level's last op: so are the later ones
+          const uint32_t op = (word >> (j4 + 1)) & 0x7u;
+          // palette cursor: Pa ops up to this one; Pd: the delta nibble dn
+          // after it selects cursor - dn - 1
+          const uint32_t ipj = ip_start + __popc(paN & (below | (1u << j4)));
+          const bool isd = (op == FCS_OP_PD);
+          const uint32_t dn = ((j4 < 28u) ? (word >> (j4 + 4u)) : wnext) & 0xFu;
+          const uint32_t pidx = ipj - (isd ? dn + 1u : 0u);  // wraps past P when the delta reaches before the palette
+          e_pal = e_pal || pidx >= P;
+          e_ovr = e_ovr || (isd && g0 + (j4 >> 2) + 1u >= ne);
+          const uint32_t pval = min(pidx, P - 1);
+          const uint32_t m = ((kj >> 3) != q0) ? m1 : m0;
+          const uint32_t n = (m << 3) | (kj & 7u);
+          if constexpr (MODE == MODE_LEVEL) {
+            io.vnext[n] = (uint16_t)pval;
+          } else if constexpr (MODE == MODE_COMPACT) {
+            io.opv[kj] = (uint16_t)pval;
+          } else {
+            io.olev[n] = (LT)pval;
+          }
+        }
+      }
+      st |= (e_pal ? FCS_ST_PALETTE_RANGE : 0u) | (e_ovr ? FCS_ST_STREAM_OVERRUN : 0u) | (e_op ? FCS_ST_BAD_OP : 0u);
       if (k0 < K && k >= K) {
         // this lane holds the level's last op: the next level starts after it
         // (and its delta); the palette cursor there
