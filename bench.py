@@ -245,8 +245,10 @@ def copy_calibration(nbytes):
 def bench_config3(args):
     """2048^3 u32 volume (seed 2, 256^3 regions: background / ~50 cells / ~4000
     noisy cells), 32^3 bricks (262,144): step = rANS stage + decode of every
-    brick at its finest LOD (8.59 G labels, 34.4 GB). Also times one batched
-    launch of 100k shuffled (brick, LOD uniform in 0..5) requests."""
+    brick at its finest LOD (8.59 G labels, 34.4 GB), the two stages overlapped
+    (fcs_decode_volume); each stage is also timed on its own for the roofline.
+    Also times one batched launch of 100k shuffled (brick, LOD uniform in 0..5)
+    requests."""
     import numpy as np
 
     import fcs
@@ -264,13 +266,20 @@ def bench_config3(args):
         raise RuntimeError("corrupt bricks")
     tune = ctx.timing()
     vol.cache_reserve(nb * (1 << (3 * L)))
+    vol.decode_all(lod=L, pyramid=False)  # the op decoder's split autotune runs on this first full batch
 
     def step():
+        st = vol.decode_volume(lod=L, pyramid=False)
+        if np.any(st):
+            raise RuntimeError("corrupt bricks")
+        td = ctx.timing()
+        return td["pipeline_ms"], td["decoded_voxels"], td["decode_bytes"]
+
+    def stages():
         vol.decode_rans()
         tr = ctx.timing()["rans_ms"]
         vol.decode_all(lod=L, pyramid=False)
-        td = ctx.timing()
-        return tr, td["decode_ms"], td["decoded_voxels"], td["decode_bytes"]
+        return tr, ctx.timing()["decode_ms"]
 
     for _ in range(args.warmup):
         step()
@@ -281,9 +290,11 @@ def bench_config3(args):
         recs.append(step())
     ctx.sync()
     wall = time.perf_counter() - t1
-    rans_ms = float(np.mean([r[0] for r in recs]))
-    ops_ms = float(np.mean([r[1] for r in recs]))
-    voxels, ops_bytes = recs[0][2], recs[0][3]
+    dec_ms = float(np.mean([r[0] for r in recs]))
+    voxels, ops_bytes = recs[0][1], recs[0][2]
+    srec = [stages() for _ in range(max(1, args.steps))]
+    rans_ms = float(np.mean([r[0] for r in srec]))
+    ops_ms = float(np.mean([r[1] for r in srec]))
     rans_bytes = info["compressed_bytes"] + info["total_symbols"] // 2 + nb * 32
     # batched requests
     rng = np.random.default_rng(2)
@@ -301,7 +312,6 @@ def bench_config3(args):
         bms.append(ctx.timing()["decode_ms"])
     batch_ms = float(np.median(bms))
     tops = ctx.timing()
-    dec_ms = rans_ms + ops_ms
     ach = ops_bytes / (ops_ms * 1e-3) / 1e9
     out = {
         "metric": "decoded Gvoxels/s (full finest-LOD decode)",
@@ -311,9 +321,11 @@ def bench_config3(args):
         "ms_per_step": wall * 1000.0 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"{gen.size}^3 u32 regions volume (seed 2), 32^3 bricks x{nb}: rANS + finest-LOD "
-                               f"decode of every brick", "compressed_bytes": info["compressed_bytes"],
+                               f"decode of every brick, the two stages overlapped",
+                   "compressed_bytes": info["compressed_bytes"],
                    "symbols": info["total_symbols"]},
         "decode_ms": dec_ms, "rans_ms": rans_ms, "ops_ms": ops_ms,
+        "stages_back_to_back_gvox_s": voxels / ((rans_ms + ops_ms) * 1e-3) / 1e9,
         "ops_only_gvox_s": voxels / (ops_ms * 1e-3) / 1e9,
         # the op stage: the head pass (levels up to 3, one wave per request)
         # and the op kernel, timed together by the library's events

@@ -59,6 +59,8 @@ struct fcs_volume {
   RansResult* d_rres = nullptr;
   uint32_t* d_rtab = nullptr;  // x -> (symbol << 25) | (freq << 12) | (x - cum)
   uint32_t* d_rlist = nullptr; // bricks by nibble count, longest first (the rANS launch order)
+  std::vector<uint32_t> rlist; // host copy
+  uint32_t n_long = 0;         // leading entries of rlist at least half as long as the longest
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
   int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
@@ -221,15 +223,12 @@ int ops_split_of(const fcs_volume* v) {
   return v->ops_tuned ? v->ops_tuned : FCS_OPS_SPLIT_3WG;
 }
 
-int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, hipStream_t stream,
-                  bool timed, int split = 0) {
+// Sort `reqs` into lane `ln` and copy them to the device on `stream`.
+int stage_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, hipStream_t stream) {
   fcs_ctx* c = v->ctx;
   ln.reqs = reqs;
   const size_t n = reqs.size();
   if (n == 0) return 0;
-  if (split == 0) split = ops_split_of(v);
-  const int ksplit = split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG;
-  c->timing.ops_split = (uint32_t)split;
   if (int rc = ensure_req(v, ln, n)) return rc;
   // Bin the requests by estimated cost, heaviest first (stable within a bin):
   // the persistent decoder deals the front half round-robin and claims the
@@ -254,6 +253,19 @@ int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& r
   ln.sorted.resize(n);
   for (size_t i = 0; i < n; ++i) ln.sorted[i] = reqs[ln.perm[i]];
   HIP_TRY(c, hipMemcpyAsync(ln.d_req, ln.sorted.data(), n * sizeof(DecodeReq), hipMemcpyHostToDevice, stream));
+  return 0;
+}
+
+// Launch the op decoder over lane `ln`'s staged requests on `stream`; with
+// `copy_status` the statuses come back to the host in the same stream order
+// (a copy to pageable memory, which holds the host until the launch is done).
+int issue_decode(fcs_volume* v, DecodeLane& ln, hipStream_t stream, bool timed, int split, bool copy_status) {
+  fcs_ctx* c = v->ctx;
+  const size_t n = ln.reqs.size();
+  if (n == 0) return 0;
+  if (split == 0) split = ops_split_of(v);
+  const int ksplit = split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG;
+  c->timing.ops_split = (uint32_t)split;
   if (timed) HIP_TRY(c, hipEventRecord(c->ev[0], stream));
   if (v->label_bits == 32) {
     OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, ln.d_req,
@@ -266,9 +278,16 @@ int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& r
   }
   if (timed) HIP_TRY(c, hipEventRecord(c->ev[1], stream));
   ln.status_sorted.resize(n);
-  HIP_TRY(c, hipMemcpyAsync(ln.status_sorted.data(), ln.d_status, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            stream));
+  if (copy_status)
+    HIP_TRY(c, hipMemcpyAsync(ln.status_sorted.data(), ln.d_status, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              stream));
   return 0;
+}
+
+int launch_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, hipStream_t stream,
+                  bool timed, int split = 0) {
+  if (int rc = stage_decode(v, ln, reqs, stream)) return rc;
+  return issue_decode(v, ln, stream, timed, split, true);
 }
 
 // Statuses of lane `ln`'s last launch in the caller's order (the launch's
@@ -277,6 +296,24 @@ void finish_decode(DecodeLane& ln, std::vector<uint32_t>& status) {
   const size_t n = ln.reqs.size();
   status.assign(n, 0);
   for (size_t i = 0; i < n; ++i) status[ln.perm[i]] = ln.status_sorted[i];
+}
+
+// Labels and algorithmic bytes of a decode of `reqs` into the context's
+// timing record; returns the labels.
+uint64_t account_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs) {
+  uint64_t vox = 0, bytes = 0;
+  for (const auto& q : reqs) {
+    const uint32_t t = q.lod & 0xFF;
+    const uint64_t n = (q.lod & FCS_REQ_PYRAMID) ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
+    vox += n;
+    const DevBrick& bk = v->hb[q.brick < v->nb ? q.brick : 0];
+    // algorithmic bytes: nibble stream consumed (0.5 B/nibble) + palette +
+    // 48 B of brick records + labels written
+    bytes += (uint64_t)(bk.nsym + 1) / 2 + 8ull * bk.P + 48 + n * v->lbytes();
+  }
+  v->ctx->timing.decoded_voxels = vox;
+  v->ctx->timing.decode_bytes = bytes;
+  return vox;
 }
 
 // Smallest batch the op decoder's split autotuner measures on.
@@ -317,18 +354,7 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
   }
   finish_decode(v->lane_main, status);
   c->timing.decode_ms = ms;
-  uint64_t vox = 0, bytes = 0;
-  for (const auto& q : reqs) {
-    const uint32_t t = q.lod & 0xFF;
-    const uint64_t n = (q.lod & FCS_REQ_PYRAMID) ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
-    vox += n;
-    const DevBrick& bk = v->hb[q.brick < v->nb ? q.brick : 0];
-    // algorithmic bytes: nibble stream consumed (0.5 B/nibble) + palette +
-    // 48 B of brick records + labels written
-    bytes += (uint64_t)(bk.nsym + 1) / 2 + 8ull * bk.P + 48 + n * v->lbytes();
-  }
-  c->timing.decoded_voxels = vox;
-  c->timing.decode_bytes = bytes;
+  const uint64_t vox = account_decode(v, reqs);
   if (voxels_out) *voxels_out = vox;
   return 0;
 }
@@ -718,6 +744,9 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return v->hb[x].nsym > v->hb[y].nsym; });
     if (int rc2 = dev_upload(c, &v->d_rlist, order.data(), order.size() * 4)) { fcs_volume_free(v); return rc2; }
+    const uint64_t longest = v->hb[order[0]].nsym;
+    while (v->n_long < v->nb && 2ull * v->hb[order[v->n_long]].nsym >= longest) ++v->n_long;
+    v->rlist = std::move(order);
   }
   v->pal_host = std::move(hp);
   {
@@ -833,10 +862,8 @@ int fcs_volume_get_info(const fcs_volume* v, fcs_volume_info* o) {
   return 0;
 }
 
-int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
-  if (!v || !v->ctx) return FCS_E_INVALID;
-  fcs_ctx* c = v->ctx;
-  HIP_TRY(c, hipSetDevice(c->device));
+// rANS launch over every brick, in launch order (longest first)
+RansArgs rans_args(const fcs_volume* v) {
   RansArgs a{};
   a.bricks = v->d_bricks;
   a.stream = v->d_stream;
@@ -848,6 +875,31 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   a.nib = v->d_nib;
   a.res = v->d_rres;
   a.table = v->d_rtab;
+  return a;
+}
+
+// Per-brick rANS results of the last launch (on the context stream, which
+// must have every rANS launch queued) into the volume; true if any failed.
+int settle_rans(fcs_volume* v, uint32_t* out_status, bool* bad) {
+  fcs_ctx* c = v->ctx;
+  std::vector<RansResult> rr(v->nb);
+  HIP_TRY(c, hipMemcpyAsync(rr.data(), v->d_rres, rr.size() * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *bad = false;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    v->rans_status[i] = rr[i].status;
+    if (out_status) out_status[i] = rr[i].status;
+    *bad |= rr[i].status != 0;
+  }
+  v->rans_ready = true;
+  return 0;
+}
+
+int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  RansArgs a = rans_args(v);
   auto timed = [&](int mode, float* ms) -> int {
     HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(c, fcs_launch_rans(a, mode, c->stream));
@@ -887,19 +939,11 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   float ms = 0;
   if (int rc = timed(mode, &ms)) return rc;
   c->timing.rans_mapping = (uint32_t)mode;
-  std::vector<RansResult> rr(v->nb);
-  HIP_TRY(c, hipMemcpyAsync(rr.data(), v->d_rres, rr.size() * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  bool bad = false;
+  if (int rc = settle_rans(v, out_status, &bad)) return rc;
   c->timing.rans_ms = ms;
   c->timing.rans_symbols = v->symbols;
   c->timing.rans_bytes = v->compressed;
-  bool bad = false;
-  for (uint32_t i = 0; i < v->nb; ++i) {
-    v->rans_status[i] = rr[i].status;
-    if (out_status) out_status[i] = rr[i].status;
-    bad |= rr[i].status != 0;
-  }
-  v->rans_ready = true;
   if (bad) return fail(c, FCS_E_CORRUPT, "one or more brick streams failed rANS / structure checks");
   return 0;
 }
@@ -1016,22 +1060,15 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
   return 0;
 }
 
-int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
-  if (!v || !v->ctx) return FCS_E_INVALID;
-  fcs_ctx* c = v->ctx;
-  HIP_TRY(c, hipSetDevice(c->device));
-  if (int rc = settle_prefetch(v)) return rc;
-  if (!v->rans_ready) {
-    int rc = fcs_decode_rans(v, nullptr);
-    if (rc && rc != FCS_E_CORRUPT) return rc;
-  }
+// Cache slots and requests for every brick at `lod` (brick i at slot i).
+int full_requests(fcs_volume* v, uint32_t lod, int pyramid, std::vector<DecodeReq>& reqs) {
   const uint32_t t = std::min(lod, v->L);
   const uint64_t per = pyramid ? fcs_pyramid_level_offset(t + 1) : (1ull << (3 * t));
   if (!v->d_cache || (v->direct ? v->stride < per : v->cap < per * v->nb)) {
     if (int rc = fcs_cache_reserve(v, per * v->nb)) return rc;
   }
   fcs_cache_clear(v);
-  std::vector<DecodeReq> reqs(v->nb);
+  reqs.resize(v->nb);
   const uint64_t slot = v->direct ? v->stride : per;
   for (uint32_t i = 0; i < v->nb; ++i) {
     const uint64_t off = slot * i;
@@ -1042,6 +1079,20 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
     ++v->res_epoch;
   }
   v->bump = per * v->nb;
+  return 0;
+}
+
+int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
+  if (!v->rans_ready) {
+    int rc = fcs_decode_rans(v, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+  }
+  std::vector<DecodeReq> reqs;
+  if (int rc = full_requests(v, lod, pyramid, reqs)) return rc;
   std::vector<uint32_t> rs;
   if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
   uint32_t bad = 0, first = ~0u, any = 0;
@@ -1058,6 +1109,88 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
                   snprintf(b, sizeof b, "%x", any);
                   return std::string(b);
                 }() + ", first brick " + std::to_string(first) + ")");
+  return 0;
+}
+
+// Both stages for every brick, overlapped. The streams at least half as long
+// as the longest go through rANS and the op decoder on the context stream; the
+// rest on the aux stream, whose rANS and op decode run while the long streams
+// are still in their rANS stage (a stream's rANS time grows with its length;
+// the op decode of a short brick does not wait for the long ones). Bricks
+// above 32^3 decode with a workgroup that takes a whole CU's LDS, which cannot
+// start beside the rANS launch: there both stages run on the context stream.
+int fcs_decode_volume(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_status) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_prefetch(v)) return rc;
+  int mode = c->rans_mapping;
+  if (mode == FCS_RANS_MAPPING_AUTO) {
+    if (!v->rans_tuned) {
+      // the volume's first rANS pass picks the mapping
+      const int rc = fcs_decode_rans(v, nullptr);
+      if (rc && rc != FCS_E_CORRUPT) return rc;
+    }
+    mode = v->rans_tuned;
+  }
+  std::vector<DecodeReq> reqs;
+  if (int rc = full_requests(v, lod, pyramid, reqs)) return rc;
+  const uint32_t nl = v->b <= 32 ? v->n_long : v->nb;
+  std::vector<DecodeReq> rl, rs;
+  rl.reserve(nl);
+  rs.reserve(v->nb - nl);
+  for (uint32_t k = 0; k < v->nb; ++k) (k < nl ? rl : rs).push_back(reqs[v->rlist[k]]);
+  // requests go up before any launch: copies from pageable memory would
+  // otherwise hold the host until the stream before them has drained
+  if (int rc = stage_decode(v, v->lane_main, rl, c->stream)) return rc;
+  if (int rc = stage_decode(v, v->lane_aux, rs, c->aux)) return rc;
+  const RansArgs a = rans_args(v);
+  RansArgs al = a, as = a;
+  al.nlist = nl;
+  as.list = v->d_rlist + nl;
+  as.nlist = v->nb - nl;
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  HIP_TRY(c, hipEventRecord(c->main_mark, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->aux, c->main_mark, 0));
+  HIP_TRY(c, fcs_launch_rans(as, mode, c->aux));
+  if (int rc = issue_decode(v, v->lane_aux, c->aux, false, 0, false)) return rc;
+  HIP_TRY(c, hipEventRecord(c->aux_done, c->aux));
+  HIP_TRY(c, fcs_launch_rans(al, mode, c->stream));
+  if (int rc = issue_decode(v, v->lane_main, c->stream, false, 0, false)) return rc;
+  HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  for (DecodeLane* ln : {&v->lane_main, &v->lane_aux})
+    if (!ln->reqs.empty())
+      HIP_TRY(c, hipMemcpyAsync(ln->status_sorted.data(), ln->d_status, ln->reqs.size() * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, c->stream));
+  bool rbad = false;
+  if (int rc = settle_rans(v, nullptr, &rbad)) return rc;  // synchronises the context stream
+  float ms = 0;
+  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->timing.pipeline_ms = ms;
+  c->timing.rans_mapping = (uint32_t)mode;
+  account_decode(v, reqs);
+  std::vector<uint32_t> sl, ss;
+  finish_decode(v->lane_main, sl);
+  finish_decode(v->lane_aux, ss);
+  v->lane_aux.reqs.clear();
+  uint32_t nbad = 0, any = 0, first = ~0u;
+  for (uint32_t k = 0; k < v->nb; ++k) {
+    const uint32_t b = v->rlist[k];
+    const uint32_t st = v->rans_status[b] | (k < nl ? sl[k] : ss[k - nl]);
+    if (out_status) out_status[b] = st;
+    if (st) {
+      ++nbad;
+      any |= st;
+      first = std::min(first, b);
+    }
+  }
+  if (nbad) {
+    char bits[16];
+    snprintf(bits, sizeof bits, "%x", any);
+    return fail(c, FCS_E_CORRUPT, std::to_string(nbad) + " bricks failed decoding (status bits 0x" + bits +
+                                      ", first brick " + std::to_string(first) + ")");
+  }
   return 0;
 }
 

@@ -114,7 +114,7 @@ class TimingStruct(ctypes.Structure):
         ("tune_ops_4wg_ms", ctypes.c_float),
         ("tune_win_ms", ctypes.c_float),
         ("tune_win_ballot_ms", ctypes.c_float),
-        ("pad", ctypes.c_uint32),
+        ("pipeline_ms", ctypes.c_float),
     ]
 
 
@@ -169,6 +169,7 @@ _sig("fcs_cache_clear", ctypes.c_int, c_vp)
 _sig("fcs_cache_reserve_direct", ctypes.c_int, c_vp)
 _sig("fcs_request_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32, c_u64p, c_u32p)
 _sig("fcs_decode_all", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_int)
+_sig("fcs_decode_volume", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_int, c_u32p)
 _sig("fcs_read_brick", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, c_vp)
 _sig("fcs_read_cache", ctypes.c_int, c_vp, ctypes.c_uint64, ctypes.c_uint64, c_vp)
 _sig("fcs_read_nibbles", ctypes.c_int64, c_vp, ctypes.c_uint32, c_u8p, ctypes.c_uint64)

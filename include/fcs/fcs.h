@@ -11,6 +11,8 @@
  *                            (web/scripts/cacheProcessing.js:303-322, web/scripts/gpuPasses.js:15-30)
  *                            over the decodeBrick kernel (web/shaders/decode.wgsl:344-508)
  *   fcs_decode_all        <- initializeCache(forceAll) (web/scripts/renderLoop.js:134-181)
+ *   fcs_decode_volume     <- decodeAllRansBricks followed by initializeCache(forceAll) (app.js:75-82 then
+ *                            renderLoop.js:134-181), the two stages overlapped
  *   fcs_read_brick        <- decodeBrick<b> (encoder/CSVEncoder/src/brickDecoding.hpp:174-242)
  *   fcs_render            <- frame(): LOD update + decode + rayTrace pass
  *                            (web/scripts/renderLoop.js:212-359, web/shaders/rayTrace.wgsl:565-906)
@@ -140,7 +142,7 @@ typedef struct {
   float tune_ops_4wg_ms;  /* autotuner: the same batch, 4 workgroups/CU */
   float tune_win_ms;      /* autotuner: whole-volume rANS time, one wave per brick, scalar byte window (table) */
   float tune_win_ballot_ms; /* autotuner: same, ballot over a per-lane CDF */
-  uint32_t pad;
+  float pipeline_ms;      /* last fcs_decode_volume: both stages, overlapped */
 } fcs_timing;
 
 /* Options (fcs_set_option). */
@@ -197,6 +199,15 @@ int fcs_request_bricks(fcs_volume* vol, const uint32_t* brick, const uint8_t* lo
 /* Decode every brick at LOD `lod` (lod > levels: every brick at its finest LOD).
  * pyramid != 0 writes all levels 0..lod of each brick (coarse-first layout). */
 int fcs_decode_all(fcs_volume* vol, uint32_t lod, int pyramid);
+
+/* Every brick's rANS stage and then its decode at LOD `lod`, as fcs_decode_rans
+ * followed by fcs_decode_all, with the stages overlapped: the bricks whose
+ * streams are at least half as long as the longest run on the context stream,
+ * the others on a second stream, where their op decode proceeds while the long
+ * streams are still in the rANS stage. Same labels and statuses as the two
+ * calls. out_status (nullable, num_bricks entries): per brick, the FCS_ST_*
+ * bits of both stages. fcs_get_timing: pipeline_ms. */
+int fcs_decode_volume(fcs_volume* vol, uint32_t lod, int pyramid, uint32_t* out_status);
 
 /* Copy the labels of `brick` at `lod` to the host (8^lod labels, Morton order),
  * decoding it first if it is not resident. */

@@ -1,0 +1,122 @@
+"""fcs_decode_volume: the rANS stage and the full decode in one call, the long
+streams on the context stream and the rest on a second stream, overlapped.
+Bit-exact against decode_rans + decode_all (and the oracle): pyramids and
+single levels, both rANS mappings, both label widths, corrupt streams, and a
+512^3 volume of 64^3 bricks whose stream lengths really split."""
+import numpy as np
+import pytest
+
+import fcs
+from conftest import golden_bytes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["cfg1_b32", "rand_b4", "rand_b8", "const_mix_b8", "checker_b16", "u64_b8", "microns8_b64"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if fcs.device_count() == 0:
+        pytest.skip("no HIP device")
+    c = fcs.Context(0)
+    yield c
+    c.close()
+
+
+def _full(ctx, data, b, pipelined, lod=255, pyramid=True, mapping=0):
+    ctx.set_rans_mapping(mapping)
+    vol = ctx.load(data, b)
+    try:
+        L, nb = vol.info["levels"], vol.info["num_bricks"]
+        t = min(lod, L)
+        per = O.pyramid_offset(t + 1) if pyramid else 1 << (3 * t)
+        if pipelined:
+            st = vol.decode_volume(lod, pyramid)
+            assert ctx.timing()["pipeline_ms"] > 0
+        else:
+            st = vol.decode_rans()
+            vol.decode_all(lod, pyramid)
+        return st.copy(), vol.read_cache(0, per * nb).reshape(nb, per).copy()
+    finally:
+        vol.close()
+        ctx.set_rans_mapping(0)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_decode_volume_pyramids_match_two_calls_and_oracle(ctx, manifest, name):
+    print(f"[gpu] decode_volume pyramids {name}", flush=True)
+    b = manifest[name]["brick_size"]
+    data = golden_bytes(name)
+    st, got = _full(ctx, data, b, True)
+    _, want = _full(ctx, data, b, False)
+    assert np.all(st == 0)
+    assert np.array_equal(got, want)
+    ds = O.Dataset(data, b)
+    for i in range(0, ds.num_bricks, max(1, ds.num_bricks // 6)):
+        for t in range(ds.L + 1):
+            lab, s = ds.decode(i, t)
+            assert s == 0
+            seg = got[i, O.pyramid_offset(t):O.pyramid_offset(t + 1)].astype(np.uint64)
+            assert np.array_equal(seg, lab), (name, i, t)
+
+
+@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("name", ["rand_b8", "microns8_b64"])
+def test_decode_volume_forced_mappings(ctx, manifest, name, mapping):
+    print(f"[gpu] decode_volume {name} mapping {mapping}", flush=True)
+    b = manifest[name]["brick_size"]
+    data = golden_bytes(name)
+    L = O.Dataset(data, b).L
+    for lod, pyr in ((L, False), (max(L - 1, 0), True)):
+        st, got = _full(ctx, data, b, True, lod, pyr, mapping)
+        _, want = _full(ctx, data, b, False, lod, pyr, mapping)
+        assert np.all(st == 0)
+        assert np.array_equal(got, want), (name, mapping, lod, pyr)
+
+
+def test_decode_volume_corrupt_streams(ctx):
+    """Damaged streams: per-brick statuses are the rANS stage's bits or'ed with
+    the op decoder's, and every brick's labels equal the two-call path's."""
+    print("[gpu] decode_volume corrupt streams", flush=True)
+    data = bytearray(golden_bytes("rand_b8"))
+    ds = O.Dataset(bytes(data), 8)
+    for i in range(0, ds.num_bricks, 3):
+        br = ds.brick(i)
+        if br.E > 8:
+            data[br.str_off + br.E // 3] ^= 0x5A
+    data = bytes(data)
+    L, nb = ds.L, ds.num_bricks
+    n = 1 << (3 * L)
+    vol = ctx.load(data, 8)
+    st = vol.decode_volume(L, False, raise_on_error=False)
+    got = vol.read_cache(0, n * nb).reshape(nb, n).copy()
+    vol.close()
+    vol = ctx.load(data, 8)
+    rst = vol.decode_rans(raise_on_error=False)
+    vol.cache_reserve(nb * n)
+    off, ost = vol.request_bricks(np.arange(nb, dtype=np.uint32), np.full(nb, L, np.uint8), raise_on_error=False)
+    want = np.stack([vol.read_cache(int(o), n) for o in off])
+    vol.close()
+    assert np.any(st != 0)
+    assert np.array_equal(st, rst | ost)
+    assert np.array_equal(got, want)
+
+
+def test_decode_volume_512cube_b64_splits(ctx):
+    """512^3 Voronoi volume, 64^3 bricks (512 streams of very different
+    lengths): the pipelined pyramid equals the two-call one; oracle spot checks."""
+    print("[gpu] decode_volume 512^3 b64", flush=True)
+    data = fcs.encode_generated(fcs.Generator(2), 64)
+    ds = O.Dataset(data)
+    ns = np.array([ds.brick(i).nsym for i in range(ds.num_bricks)])
+    n_long = int(np.sum(2 * ns >= ns.max()))
+    assert 0 < n_long < ds.num_bricks
+    st, got = _full(ctx, data, 0, True)
+    _, want = _full(ctx, data, 0, False)
+    assert np.all(st == 0)
+    assert np.array_equal(got, want)
+    for i in (int(np.argmax(ns)), int(np.argmin(ns)), 100, 511):
+        lab, s = ds.decode(i, ds.L)
+        assert s == 0
+        assert np.array_equal(got[i, O.pyramid_offset(ds.L):].astype(np.uint64), lab)
