@@ -228,6 +228,29 @@ static napi_value js_decode_all(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* decodeVolume(volume, lod = 255, pyramid = true) -> Uint32Array status per
+ * brick: decodeAllRansBricks then decodeAll, the two stages overlapped
+ * (fcs_decode_volume) */
+static napi_value js_decode_volume(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  uint32_t lod = 255;
+  bool pyr = true;
+  napi_valuetype vt;
+  napi_typeof(env, argv[1], &vt);
+  if (vt == napi_number) get_u32(env, argv[1], &lod);
+  napi_typeof(env, argv[2], &vt);
+  if (vt == napi_boolean) napi_get_value_bool(env, argv[2], &pyr);
+  void* st;
+  napi_value arr = new_typed(env, napi_uint32_array, vb->info.num_bricks, 4, &st);
+  if (!arr) return NULL;
+  int rc = fcs_decode_volume(vb->vol, lod, pyr ? 1 : 0, (uint32_t*)st);
+  if (rc < 0 && rc != FCS_E_CORRUPT) return throw_fcs(env, rc, vb->ctx, "fcs_decode_volume");
+  return arr;
+}
+
 /* readBrick(volume, brick, lod) -> Uint32Array | BigUint64Array (fcs_read_brick) */
 static napi_value js_read_brick(napi_env env, napi_callback_info info) {
   napi_value argv[3];
@@ -457,6 +480,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"decodeAllRansBricks", NULL, js_decode_rans, NULL, NULL, NULL, napi_default, NULL},
       {"requestBricks", NULL, js_request, NULL, NULL, NULL, napi_default, NULL},
       {"decodeAll", NULL, js_decode_all, NULL, NULL, NULL, napi_default, NULL},
+      {"decodeVolume", NULL, js_decode_volume, NULL, NULL, NULL, napi_default, NULL},
       {"readBrick", NULL, js_read_brick, NULL, NULL, NULL, napi_default, NULL},
       {"orbitCamera", NULL, js_orbit, NULL, NULL, NULL, napi_default, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},

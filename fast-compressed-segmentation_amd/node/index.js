@@ -76,6 +76,7 @@ module.exports = {
   decodeAllRansBricks,
   requestBricks: addon.requestBricks,
   decodeAll: addon.decodeAll,
+  decodeVolume: addon.decodeVolume,
   readBrick: addon.readBrick,
   orbitCamera: addon.orbitCamera,
   render: addon.render,

@@ -18,6 +18,14 @@ for (let b = 0; b < h.numBricks; b++) {
   for (let t = 0; t <= h.levels; t++) parts.push(Buffer.from(fcs.readBrick(ds, b, t).buffer));
 }
 fs.writeFileSync(path.join(outDir, 'bricks.bin'), Buffer.concat(parts));
+// both stages again in one call (overlapped), then every brick once more
+const vst = fcs.decodeVolume(ds);
+fs.writeFileSync(path.join(outDir, 'volume_status.bin'), Buffer.from(vst.buffer));
+const parts2 = [];
+for (let b = 0; b < h.numBricks; b++) {
+  for (let t = 0; t <= h.levels; t++) parts2.push(Buffer.from(fcs.readBrick(ds, b, t).buffer));
+}
+fs.writeFileSync(path.join(outDir, 'bricks_after_volume.bin'), Buffer.concat(parts2));
 const g = h.gridSize;
 const cam = { position: [g / 2, g / 2, -1], forward: [0, 0, 1], right: [1, 0, 0], up: [0, -1, 0],
               tanHalfFov: 0, orthoHalfHeight: g / 2 };

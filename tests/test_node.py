@@ -17,7 +17,7 @@ ADDON = os.path.join(ROOT, "fast-compressed-segmentation_amd", "node", "fcs.node
 pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the addon missing")
 
 API = ["deviceCount", "createContext", "loadDatasetFromArrayBuffer", "decodeAllRansBricks", "requestBricks",
-       "decodeAll", "readBrick", "orbitCamera", "render", "updateLightingOptions", "applyLabelVisibility",
+       "decodeAll", "decodeVolume", "readBrick", "orbitCamera", "render", "updateLightingOptions", "applyLabelVisibility",
        "applyLabelHover", "getDatasetLabels", "transferFunction"]
 
 
@@ -42,6 +42,10 @@ def test_node_path_bit_exact(tmp_path):
     assert meta["badMagicThrows"] is True
     st = np.fromfile(tmp_path / "rans_status.bin", np.uint32)
     assert st.shape == (64,) and np.all(st == 0)
+    vst = np.fromfile(tmp_path / "volume_status.bin", np.uint32)
+    assert vst.shape == (64,) and np.all(vst == 0)
+    assert np.array_equal(np.fromfile(tmp_path / "bricks_after_volume.bin", np.uint32),
+                          np.fromfile(tmp_path / "bricks.bin", np.uint32))
     ds = O.Dataset(golden_bytes("cfg1_b32"), 32)
     got = np.fromfile(tmp_path / "bricks.bin", np.uint32)
     pos = 0
