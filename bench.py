@@ -324,7 +324,7 @@ def bench_config3(args):
                                f"decode of every brick, the two stages overlapped",
                    "compressed_bytes": info["compressed_bytes"],
                    "symbols": info["total_symbols"]},
-        "decode_ms": dec_ms, "rans_ms": rans_ms, "ops_ms": ops_ms,
+        "decode_ms": dec_ms, "decode_ms_steps": [round(r[0], 3) for r in recs], "rans_ms": rans_ms, "ops_ms": ops_ms,
         "stages_back_to_back_gvox_s": voxels / ((rans_ms + ops_ms) * 1e-3) / 1e9,
         "ops_only_gvox_s": voxels / (ops_ms * 1e-3) / 1e9,
         # the op stage: the head pass (levels up to 3, one wave per request)

@@ -451,7 +451,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         pv1 = io.vcur[m1];
       }
       bool e_pal = false, e_ovr = false, e_op = false;
-#pragma unroll 2
+#pragma unroll
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t v = (word >> (4 * j)) & 0xFu;
         const uint32_t op = v >> 1;
