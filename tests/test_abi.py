@@ -45,3 +45,15 @@ def test_no_device_is_reported_not_faked():
     with pytest.raises(fcs.FcsError) as e:
         fcs.Context(0)
     assert e.value.code == -5  # FCS_E_NODEVICE
+
+
+def test_decode_entry_points_reject_a_null_volume():
+    """The decode entry points check their volume before any device call."""
+    import fcs
+
+    lib = ctypes.CDLL(fcs.lib_path)
+    for name, args in (("fcs_decode_rans", (None, None)), ("fcs_decode_all", (None, 5, 1)),
+                       ("fcs_decode_volume", (None, 5, 1, None))):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        assert fn(*args) == -1, name  # FCS_E_INVALID
