@@ -83,8 +83,8 @@ def main():
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU sharding is "
-                                   "described in DESIGN.md and not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; sharding the volume across "
+                                   "GPUs is not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
