@@ -14,8 +14,8 @@
  *                      decodeBrick<b> at the finest LOD; LOD 0 == palette[0].
  *   or_render          orthographic +z frames == brute-force column scan;
  *                      perspective frames: no independent pin beyond the
- *                      symmetric/axis-aligned cases in the tests (parity
- *                      partially pinned).
+ *                      symmetric/axis-aligned cases in the tests: parity
+ *                      partially pinned for perspective frames.
  *   or_brick_lod       exact thresholds of camera.js:103-137 in tests.
  *   or_render_rgba     closed forms: a one-label volume seen along +z
  *                      composites to 1-(1-a)^n in the label's colour; a
