@@ -142,14 +142,22 @@ __global__ __launch_bounds__(64) void rans_wave_kernel(RansArgs a) {
       const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_c, (int)sym);
       state = f * (state >> FCS_RANS_LOGM) + x - c;
     }
-    while (state < FCS_RANS_L && pos > 0) {
-      // the empty asm keeps the byte in a scalar register: (state << 8) | byte
-      // is otherwise matched as a funnel shift, which exists only on the
-      // vector unit, and each renormalisation step would round-trip through
-      // a vector register on the serial chain
-      uint32_t byte = next_byte();
-      __asm__ volatile("" : "+s"(byte));
-      state = (state << 8) | byte;
+    // the empty asm keeps the byte in a scalar register: (state << 8) | byte
+    // is otherwise matched as a funnel shift, which exists only on the vector
+    // unit, and each renormalisation step would round-trip through a vector
+    // register on the serial chain
+    auto renorm = [&]() {
+      while (state < FCS_RANS_L && pos > 0) {
+        uint32_t byte = next_byte();
+        __asm__ volatile("" : "+s"(byte));
+        state = (state << 8) | byte;
+      }
+    };
+    if constexpr (TM == 0) {
+      // most symbols take no byte: that path falls through
+      if (__builtin_expect(state < FCS_RANS_L, 0)) renorm();
+    } else {
+      renorm();
     }
     return sym;
   };
