@@ -120,3 +120,34 @@ def test_decode_volume_512cube_b64_splits(ctx):
         lab, s = ds.decode(i, ds.L)
         assert s == 0
         assert np.array_equal(got[i, O.pyramid_offset(ds.L):].astype(np.uint64), lab)
+
+
+@pytest.mark.parametrize("name", ["rand_b8", "cfg1_b32"])
+def test_decode_volume_direct_pool(ctx, manifest, name):
+    """A direct-mapped pool (brick i at slot i, slots of the finest level's
+    size): single-level decodes at the finest LOD and one coarser, as the
+    two-call path writes them."""
+    print(f"[gpu] decode_volume direct pool {name}", flush=True)
+    b = manifest[name]["brick_size"]
+    data = golden_bytes(name)
+    for lod_drop in (0, 1):
+        out = []
+        for pipelined in (True, False):
+            vol = ctx.load(data, b)
+            try:
+                vol.cache_reserve_direct()
+                L, nb = vol.info["levels"], vol.info["num_bricks"]
+                t = L - lod_drop
+                if pipelined:
+                    assert np.all(vol.decode_volume(t, False) == 0)
+                else:
+                    vol.decode_rans()
+                    vol.decode_all(t, False)
+                stride, n = 1 << (3 * L), 1 << (3 * t)
+                out.append(np.stack([vol.read_cache(i * stride, n) for i in range(nb)]))
+            finally:
+                vol.close()
+        assert np.array_equal(out[0], out[1]), (name, lod_drop)
+        ds = O.Dataset(data, b)
+        for i in (0, ds.num_bricks - 1):
+            assert np.array_equal(out[0][i].astype(np.uint64), ds.decode(i, ds.L - lod_drop)[0])
