@@ -1,20 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: brick decoder + DDA marcher on one MI355X.
 
-Workload (the 512^3 demo-scale configuration): a 512^3 uint32 label volume
-from seed 1 (anisotropic Voronoi, 20,000 cells, metric 1:1:2.5, +-2 voxel
-boundary noise, 25 % background in two blobs), 64^3 bricks (512 bricks),
-encoded to .csdb by the CPU producer and resident in HBM before timing.
+Default workload (--config 3, the headline): the 2048^3 uint32 regions volume
+(seed 2), 32^3 bricks (262,144), encoded to .csdb by the CPU producer and
+resident in HBM before timing. One step = the rANS stage + the decode of every
+brick at its finest LOD (8.59 G labels), the two stages overlapped
+(fcs_decode_volume). value = decoded Gvoxels/s (HIP events on the library
+stream). Every label of the last step is then compared with the source
+volume, a batched launch of 100k (brick, LOD) requests is timed and compared
+with the oracle, and the march over the decoded volume is timed at 1080p.
 
-One step = full decode of every brick from its compressed bytes (device rANS
-stage + device op decoder writing every LOD 0..6 of every brick), then a
-perspective DDA march at 1920x1080 for 8 orbit poses (60 deg FOV, radius 1.5x
-the volume diagonal, yaw pi/4 + k pi/4, pitch 0.15 pi) with the per-brick LOD
-rule of the reference app.
-
-value = decoded Gvoxels/s of the full decode (device time, HIP events on the
-library stream); frames_per_s_1080p = 1080p frames/s of the march. Prints one
-JSON line.
+Other workloads: --config 2 (512^3 demo scale, 64^3 bricks: full pyramid
+decode + 8 orbit poses at 1080p), --config 4 (1024^3 fly-throughs, learned
+prefetch vs on-demand), --config 5 (serving many viewers). Prints one JSON
+line.
 """
 import argparse
 import json
@@ -60,8 +59,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
-                    help="2: 512^3 demo scale (default); 3: 2048^3 full decode + 100k batched requests; "
+    ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5),
+                    help="3: 2048^3 full decode + 100k batched requests (default); 2: 512^3 demo scale; "
                          "4: 1024^3 fly-throughs, learned prefetch vs on-demand; "
                          "5: serving 64 viewers over 256^3..2048^3 volumes")
     ap.add_argument("--size", type=int, default=0, help="override the volume edge (config 3)")
@@ -243,12 +242,20 @@ def copy_calibration(nbytes):
 
 
 def bench_config3(args):
-    """2048^3 u32 volume (seed 2, 256^3 regions: background / ~50 cells / ~4000
-    noisy cells), 32^3 bricks (262,144): step = rANS stage + decode of every
-    brick at its finest LOD (8.59 G labels, 34.4 GB), the two stages overlapped
-    (fcs_decode_volume); each stage is also timed on its own for the roofline.
-    Also times one batched launch of 100k shuffled (brick, LOD uniform in 0..5)
-    requests."""
+    """The headline workload: the 2048^3 u32 regions volume (seed 2, 256^3
+    regions of background / ~50 cells / ~4000 noisy cells), 32^3 bricks
+    (262,144), encoded by the CPU producer and resident in HBM before timing.
+
+    One step = the rANS stage + the decode of every brick at its finest LOD
+    (8.59 G labels, 34.4 GB), the two stages overlapped on two streams
+    (fcs_decode_volume). value = decoded Gvoxels/s of that call (HIP events on
+    the library stream). After the timed steps every label of the last decode
+    is compared with the generator's source bricks (the finest LOD equals the
+    source volume), and one batched launch of 100k shuffled (brick, LOD uniform
+    in 0..5) requests is timed and compared with the oracle. Each stage is
+    also timed on its own for the roofline (the op stage is the dominant
+    kernel). frames_per_s_1080p: the march over the decoded volume at 1920x1080
+    for 8 orbit poses (every brick at its finest LOD, so no frame decodes)."""
     import numpy as np
 
     import fcs
@@ -261,11 +268,12 @@ def bench_config3(args):
     vol = ctx.load(data)
     info = vol.info
     nb, L = info["num_bricks"], info["levels"]
+    per = 1 << (3 * L)
     st = vol.decode_rans()
     if np.any(st):
         raise RuntimeError("corrupt bricks")
     tune = ctx.timing()
-    vol.cache_reserve(nb * (1 << (3 * L)))
+    vol.cache_reserve(nb * per)
     vol.decode_all(lod=L, pyramid=False)  # the op decoder's split autotune runs on this first full batch
 
     def step():
@@ -290,21 +298,65 @@ def bench_config3(args):
         recs.append(step())
     ctx.sync()
     wall = time.perf_counter() - t1
+    log(f"[bench] {args.steps} timed steps: {wall * 1000.0 / args.steps:.2f} ms per step")
     dec_ms = float(np.mean([r[0] for r in recs]))
     voxels, ops_bytes = recs[0][1], recs[0][2]
+
+    # every label of the last timed decode against the source volume
+    t2 = time.time()
+    off, st2 = vol.request_bricks(np.arange(nb, dtype=np.uint32), np.full(nb, L, np.uint8))
+    if np.any(st2) or not np.array_equal(off, np.arange(nb, dtype=np.uint64) * np.uint64(per)):
+        raise RuntimeError("decoded bricks are not resident in slot order")
+    bad_bricks = 0
+    chunk = 8192
+    for first in range(0, nb, chunk):
+        n = min(chunk, nb - first)
+        dev = vol.read_cache(first * per, n * per).reshape(n, per)
+        src = gen.bricks(32, first, n)
+        bad_bricks += int(np.count_nonzero(np.any(dev != src, axis=1)))
+    check_s = time.time() - t2
+    log(f"[bench] labels checked against the source volume in {check_s:.1f}s: {bad_bricks} bricks differ")
+
     srec = [stages() for _ in range(max(1, args.steps))]
     rans_ms = float(np.mean([r[0] for r in srec]))
     ops_ms = float(np.mean([r[1] for r in srec]))
     rans_bytes = info["compressed_bytes"] + info["total_symbols"] // 2 + nb * 32
-    # batched requests
+
+    # the march over the decoded volume: 8 orbit poses at 1080p, finest LOD
+    W, H = 1920, 1080
+    G = info["grid_size"]
+    radius = 1.5 * math.sqrt(3.0) * G
+    cams = [vol.orbit_camera(math.pi / 4 + k * math.pi / 4, 0.15 * math.pi, radius, math.pi / 3) for k in range(8)]
+    finest = np.full(nb, L, np.uint8)
+    vol.render(cams[0], W, H, brick_lod=finest, want=())
+    march = []
+    for cam in cams:
+        f = vol.render(cam, W, H, brick_lod=finest, want=())
+        if f.bricks_decoded:
+            raise RuntimeError("frame needed a decode after the full decode")
+        march.append(f.march_ms)
+    march_ms = float(np.mean(march))
+
+    # batched requests, checked against the oracle
+    from oracle import oracle as O
+
     rng = np.random.default_rng(2)
     nreq = min(100000, nb)
     bricks = rng.choice(nb, nreq, replace=False).astype(np.uint32)
     lods = rng.integers(0, L + 1, nreq).astype(np.uint8)
     vol.cache_clear()
-    _, bst = vol.request_bricks(bricks, lods)
+    boff, bst = vol.request_bricks(bricks, lods)
     tb = ctx.timing()
     bvox, bbytes = tb["decoded_voxels"], tb["decode_bytes"]
+    sizes = np.left_shift(np.uint64(1), np.uint64(3) * lods.astype(np.uint64))
+    bdev = vol.read_cache(0, int((boff + sizes).max()))
+    ds = O.Dataset(data)
+    threads = os.cpu_count() or 1
+    bref, broff, brst = ds.decode_requests(bricks, lods, np.uint32, threads)
+    bbad = sum(1 for k in range(nreq)
+               if not np.array_equal(bdev[int(boff[k]):int(boff[k] + sizes[k])],
+                                     bref[int(broff[k]):int(broff[k] + sizes[k])]))
+    del bdev, bref
     bms = []
     for _ in range(3):
         vol.cache_clear()
@@ -313,26 +365,36 @@ def bench_config3(args):
     batch_ms = float(np.median(bms))
     tops = ctx.timing()
     ach = ops_bytes / (ops_ms * 1e-3) / 1e9
+    pipe_ach = ops_bytes / (dec_ms * 1e-3) / 1e9
+    exact = bad_bricks == 0 and bbad == 0 and not np.any(bst)
     out = {
-        "metric": "decoded Gvoxels/s (full finest-LOD decode)",
+        "metric": "decoded Gvoxels/s + 1080p frames/s, chosen vs plain GPU form, bit-exact",
         "value": voxels / (dec_ms * 1e-3) / 1e9,
         "unit": "Gvoxels/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1000.0 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"{gen.size}^3 u32 regions volume (seed 2), 32^3 bricks x{nb}: rANS + finest-LOD "
-                               f"decode of every brick, the two stages overlapped",
-                   "compressed_bytes": info["compressed_bytes"],
-                   "symbols": info["total_symbols"]},
+                               f"decode of every brick (fcs_decode_volume, the two stages overlapped)",
+                   "compressed_bytes": info["compressed_bytes"], "symbols": info["total_symbols"],
+                   "form": "full decode"},
+        "bit_exact": bool(exact),
+        "label_check": {"labels": int(nb) * per, "bricks_differing_from_source": bad_bricks,
+                        "batched_requests_differing_from_oracle": int(bbad), "seconds": check_s},
         "decode_ms": dec_ms, "decode_ms_steps": [round(r[0], 3) for r in recs], "rans_ms": rans_ms, "ops_ms": ops_ms,
+        "decoded_voxels_per_step": int(voxels),
         "stages_back_to_back_gvox_s": voxels / ((rans_ms + ops_ms) * 1e-3) / 1e9,
         "ops_only_gvox_s": voxels / (ops_ms * 1e-3) / 1e9,
-        # the op stage: the head pass (levels up to 3, one wave per request)
-        # and the op kernel, timed together by the library's events
+        "overlapped_call_gbs": pipe_ach, "overlapped_call_frac": pipe_ach / HBM_PEAK_GBS,
+        # the dominant kernel: the op stage (head pass + op kernel), timed by
+        # the library's events on the stream it runs on
         "roofline": {"bound": "hbm", "kernel": "ops_head_kernel+ops_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic("hbm_traffic_2048cube_b32.md", "ops_stage")
-                     if gen.size == 2048 else None},
+                     if gen.size == 2048 else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, profiles/hbm_traffic_2048cube_b32.md"},
+        "frames_per_s_1080p": 1000.0 / march_ms, "rays_per_s": W * H * 1000.0 / march_ms,
+        "march_ms_per_frame": march_ms,
         "rans_gbs": rans_bytes / (rans_ms * 1e-3) / 1e9,
         "rans_autotune": {"mapping": RANS_MAPPINGS.get(tune["rans_mapping"]), "lane_ms": tune["tune_lane_ms"],
                           "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],
@@ -342,31 +404,77 @@ def bench_config3(args):
                          "3wg_ms": tops["tune_ops_3wg_ms"], "4wg_ms": tops["tune_ops_4wg_ms"]},
         "batched": {"requests": int(nreq), "lods": "uniform 0..%d" % L, "ms": batch_ms,
                     "gvox_s": bvox / (batch_ms * 1e-3) / 1e9, "gbs": bbytes / (batch_ms * 1e-3) / 1e9,
-                    "status_ok": bool(np.all(bst == 0))},
+                    "status_ok": bool(np.all(bst == 0)), "differing_from_oracle": int(bbad)},
     }
     if not args.no_cpu_baseline:
-        # the oracle restatement (rANS + finest-LOD decode per brick) on every
-        # host core, over random bricks of the same volume for a bounded time
-        from concurrent.futures import ThreadPoolExecutor
-
-        from oracle import oracle as O
-
-        threads = os.cpu_count() or 1
-        ds = O.Dataset(data)
-        sample = np.random.default_rng(5).permutation(nb)
-        done = 0
-        t2 = time.time()
-        with ThreadPoolExecutor(threads) as ex:
-            while done < nb and time.time() - t2 < args.cpu_seconds * 0.6:
-                chunk = sample[done:done + 16 * threads]
-                list(ex.map(lambda i: ds.decode(int(i), L), chunk))
-                done += len(chunk)
-        cpu_s = time.time() - t2
-        out["cpu_baseline"] = {
-            "value": done * (1 << (3 * L)) / cpu_s / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "port",
-            "sample": f"oracle rANS + finest-LOD decode of {done} random bricks of the volume on {threads} threads",
-        }
+        out["cpu_baseline"] = cpu_baseline_config3(args, data, ds, nb, L)
     print(json.dumps(out))
+
+
+def cpu_baseline_config3(args, data, ds, nb, L):
+    """Finest-LOD decode of random bricks of the volume on every host core,
+    each for about cpu_seconds / 2: the oracle restatement ("port") and, where
+    oracle/_ref was built, the reference's own decodeRansStream +
+    decodeBrick<32> ("reference", over non-constant bricks: the reference
+    decoder rejects a brick whose root stop bit is set)."""
+    import ctypes
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = os.cpu_count() or 1
+    per = 1 << (3 * L)
+    sample = np.random.default_rng(5).permutation(nb)
+    done = 0
+    t0 = time.time()
+    with ThreadPoolExecutor(threads) as ex:
+        while done < nb and time.time() - t0 < args.cpu_seconds * 0.5:
+            chunk = sample[done:done + 16 * threads]
+            list(ex.map(lambda i: ds.decode(int(i), L), chunk))
+            done += len(chunk)
+    port_s = time.time() - t0
+    res = {"value": done * per / port_s / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "port",
+           "sample": f"oracle rANS + finest-LOD decode of {done} random bricks of the volume on {threads} threads"}
+    if O.ref_available():
+        R = O.ref()
+        with tempfile.NamedTemporaryFile(suffix=".csbd") as f:
+            f.write(data)
+            f.flush()
+            n = R.ref_load(f.name.encode())
+        if n == nb:
+            # file index of every non-constant brick (palette size > 1), random order
+            pal = np.array([ds.brick(i).P for i in range(nb)])
+            file_of = np.zeros(nb, np.int64)
+            for k in range(nb):
+                file_of[R.ref_brick_id(k)] = k
+            cand = file_of[np.random.default_rng(6).permutation(np.nonzero(pal > 1)[0])].astype(np.uint32)
+            dec, pos, cs, rej = 0, 0, ctypes.c_uint64(), ctypes.c_uint32()
+            t1 = time.time()
+            while pos < cand.size and time.time() - t1 < args.cpu_seconds * 0.5:
+                part = np.ascontiguousarray(cand[pos:pos + 64 * threads])
+                got = R.ref_decode_finest_batch(part.ctypes.data_as(O.c_u32p), part.size, 32, threads,
+                                                ctypes.byref(cs), ctypes.byref(rej))
+                dec += max(0, int(got))
+                pos += part.size
+            ref_s = time.time() - t1
+            res["reference"] = {
+                "value": dec * per / ref_s / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "reference",
+                "sample": f"reference decodeRansStream + decodeBrick<32> (oracle/_ref) on {dec} random non-constant "
+                          f"bricks on {threads} threads"}
+            # the port over the same kind of bricks, for the calibration
+            nc = np.random.default_rng(6).permutation(np.nonzero(pal > 1)[0])
+            done2 = 0
+            t2 = time.time()
+            with ThreadPoolExecutor(threads) as ex:
+                while done2 < nc.size and time.time() - t2 < min(ref_s, args.cpu_seconds * 0.5):
+                    chunk = nc[done2:done2 + 16 * threads]
+                    list(ex.map(lambda i: ds.decode(int(i), L), chunk))
+                    done2 += len(chunk)
+            res["port_on_nonconstant_bricks"] = done2 * per / (time.time() - t2) / 1e9
+    return res
 
 
 def bench_config4(args):

@@ -380,9 +380,12 @@ int settle_prefetch(fcs_volume* v) {
 // rounded to float like its Float32Array upload.
 void set_lighting(fcs_volume* v, float angle_deg, float ambient, float diffuse) {
   const double a = ((double)angle_deg * 3.141592653589793) / 180.0;
-  v->light[0] = (float)std::sin(a);
-  v->light[1] = 0.6f;
-  v->light[2] = (float)-std::cos(a);
+  const float l[3] = {(float)std::sin(a), 0.6f, (float)-std::cos(a)};
+  // the values are baked into captured frames: a new epoch only when one changes
+  if (l[0] == v->light[0] && l[1] == v->light[1] && l[2] == v->light[2] && ambient == v->ambient &&
+      diffuse == v->diffuse && v->shade_epoch != 0)
+    return;
+  for (int k = 0; k < 3; ++k) v->light[k] = l[k];
   v->ambient = ambient;
   v->diffuse = diffuse;
   ++v->shade_epoch;
@@ -425,6 +428,10 @@ int ensure_tf(fcs_volume* v) {
     keys[h] = v->tf_labels[i];
     cols[h] = v->tf_cur[i];
   }
+  // Frames of any stream may be reading the table: views march on their own
+  // streams and replay graphs that hold its address. Wait for the whole
+  // device before the table is freed or overwritten.
+  HIP_TRY(c, hipDeviceSynchronize());
   if (v->tf_mask != mask || !v->d_tf_keys) {
     if (v->d_tf_keys) (void)hipFree(v->d_tf_keys);
     if (v->d_tf_rgba) (void)hipFree(v->d_tf_rgba);
@@ -435,8 +442,6 @@ int ensure_tf(fcs_volume* v) {
     v->tf_mask = mask;
     ++v->shade_epoch;
   }
-  // frames that read the table run on the context stream, synchronously
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemcpy(v->d_tf_keys, keys.data(), (size_t)cap * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(v->d_tf_rgba, cols.data(), (size_t)cap * 4, hipMemcpyHostToDevice));
   v->tf_dirty = false;
@@ -605,7 +610,7 @@ int fcs_get_timing(const fcs_ctx* c, fcs_timing* out) {
   return 0;
 }
 
-int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_volume** out) {
+static int load_csdb_body(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_volume** out) {
   if (!c || !bytes || !out) return fail(c, FCS_E_INVALID, "null argument");
   *out = nullptr;
   // magic first, with the reference loaders' message (fileIO.cpp:118-119, datasetLoader.js:76-77)
@@ -639,6 +644,9 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
   while ((uint64_t)B * B * B < v->nb) B <<= 1;
   if ((uint64_t)B * B * B != v->nb || B > 1024) return bad("brick count must be a power-of-two cube");
   v->B = B;
+  // every brick takes at least 24 bytes (ID, P, one palette entry, E,
+  // nSymbols): reject a count the input cannot hold before sizing anything by it
+  if (v->nb > (n - (16 + 8 + 128)) / 24) return bad("brick count exceeds the stream length");
   struct Raw { uint32_t id, P, E, nsym; size_t pal, str; };
   std::vector<Raw> raw(v->nb);
   size_t p = 16 + 8 + 128;
@@ -761,6 +769,10 @@ int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_
   c->volumes.push_back(v);
   *out = v;
   return 0;
+}
+
+int fcs_load_csdb(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t bso, fcs_volume** out) {
+  return fcs_guard(c, [&] { return load_csdb_body(c, bytes, n, bso, out); });
 }
 
 }  // extern "C"
@@ -895,7 +907,7 @@ int settle_rans(fcs_volume* v, uint32_t* out_status, bool* bad) {
   return 0;
 }
 
-int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
+static int decode_rans_body(fcs_volume* v, uint32_t* out_status) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -948,6 +960,10 @@ int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
   return 0;
 }
 
+int fcs_decode_rans(fcs_volume* v, uint32_t* out_status) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return decode_rans_body(v, out_status); });
+}
+
 int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
   if (!v || !v->ctx || voxels == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -984,7 +1000,7 @@ int fcs_cache_reserve_direct(fcs_volume* v) {
   return 0;
 }
 
-int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+static int request_bricks_body(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
                        uint64_t* out_off, uint32_t* out_status) {
   if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -1060,6 +1076,11 @@ int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod,
   return 0;
 }
 
+int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                       uint64_t* out_off, uint32_t* out_status) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return request_bricks_body(v, brick, lod, n, out_off, out_status); });
+}
+
 // Cache slots and requests for every brick at `lod` (brick i at slot i).
 int full_requests(fcs_volume* v, uint32_t lod, int pyramid, std::vector<DecodeReq>& reqs) {
   const uint32_t t = std::min(lod, v->L);
@@ -1082,7 +1103,7 @@ int full_requests(fcs_volume* v, uint32_t lod, int pyramid, std::vector<DecodeRe
   return 0;
 }
 
-int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
+static int decode_all_body(fcs_volume* v, uint32_t lod, int pyramid) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1112,6 +1133,10 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
   return 0;
 }
 
+int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return decode_all_body(v, lod, pyramid); });
+}
+
 // Both stages for every brick, overlapped. The streams at least half as long
 // as the longest go through rANS and the op decoder on the context stream; the
 // rest on the aux stream, whose rANS and op decode run while the long streams
@@ -1119,7 +1144,7 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
 // the op decode of a short brick does not wait for the long ones). Bricks
 // above 32^3 decode with a workgroup that takes a whole CU's LDS, which cannot
 // start beside the rANS launch: there both stages run on the context stream.
-int fcs_decode_volume(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_status) {
+static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_status) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1194,8 +1219,15 @@ int fcs_decode_volume(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_st
   return 0;
 }
 
+int fcs_decode_volume(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_status) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return decode_volume_body(v, lod, pyramid, out_status); });
+}
+
 int fcs_read_cache(fcs_volume* v, uint64_t off, uint64_t count, void* host) {
-  if (!v || !v->ctx || !host || off + count > v->cap) return FCS_E_INVALID;
+  if (!v || !v->ctx || !host) return FCS_E_INVALID;
+  // written so that it cannot wrap: a failed request's offset (~0) or a huge
+  // count is rejected, not turned into a pointer outside the pool
+  if (off > v->cap || count > v->cap - off) return fail(v->ctx, FCS_E_INVALID, "region outside the brick cache pool");
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   if (int rc = settle_prefetch(v)) return rc;
@@ -1468,7 +1500,7 @@ int decode_list(fcs_volume* v, const std::vector<uint32_t>& qb, const std::vecto
 
 extern "C" {
 
-int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+static int render_body(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
                fcs_frame* out) {
   if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -1495,7 +1527,12 @@ int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, u
   return finish_frame(v, lods, w, h, out);
 }
 
-int fcs_render_visible(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+int fcs_render(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+               fcs_frame* out) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return render_body(v, cam, brick_lod, w, h, out); });
+}
+
+static int render_visible_body(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
                        fcs_frame* out) {
   if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -1534,7 +1571,12 @@ int fcs_render_visible(fcs_volume* v, const fcs_camera* cam, const uint8_t* bric
   return finish_frame(v, lods, w, h, out);
 }
 
-int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n) {
+int fcs_render_visible(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h,
+                       fcs_frame* out) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return render_visible_body(v, cam, brick_lod, w, h, out); });
+}
+
+static int prefetch_bricks_body(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n) {
   if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1576,6 +1618,10 @@ int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod
   return (int)reqs.size();
 }
 
+int fcs_prefetch_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return prefetch_bricks_body(v, brick, lod, n); });
+}
+
 int fcs_frame_checksum(fcs_volume* v, uint64_t* out) {
   if (!v || !v->ctx || !out) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -1608,7 +1654,7 @@ int fcs_camera_features(const fcs_volume* v, const fcs_camera* cam, float* out12
   return 0;
 }
 
-int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead, uint32_t w,
+static int render_prefetch_body(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead, uint32_t w,
                         uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out, uint32_t* out_prefetched,
                         uint32_t* out_bricks) {
   if (!v || !v->ctx || !m || !cam || !out) return FCS_E_INVALID;
@@ -1646,6 +1692,12 @@ int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const 
   if (out_bricks)
     for (size_t i = 0; i < cand.size(); ++i) out_bricks[i] = resident_at(v, cand[i], cl[i]) ? cand[i] : ~0u;
   return 0;
+}
+
+int fcs_render_prefetch(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam, const fcs_camera* cam_ahead, uint32_t w,
+                        uint32_t h, float threshold, uint32_t max_prefetch, fcs_frame* out, uint32_t* out_prefetched,
+                        uint32_t* out_bricks) {
+  return fcs_guard(v ? v->ctx : nullptr, [&] { return render_prefetch_body(v, m, cam, cam_ahead, w, h, threshold, max_prefetch, out, out_prefetched, out_bricks); });
 }
 
 }  // extern "C"

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -40,6 +42,21 @@ void fcs_mlp_release_device(fcs_mlp* m);
 inline int fcs_fail(fcs_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
+}
+
+// Runs an ABI body so that no C++ exception crosses the C boundary: a failed
+// host allocation becomes FCS_E_NOMEM, anything else FCS_E_INVALID.
+template <class F>
+int fcs_guard(fcs_ctx* c, F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fcs_fail(c, FCS_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fcs_fail(c, FCS_E_INVALID, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fcs_fail(c, FCS_E_INVALID, "internal error");
+  }
 }
 
 #define FCS_HIP_TRY(ctx, expr)                                                            \

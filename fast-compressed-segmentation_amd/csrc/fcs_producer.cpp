@@ -365,6 +365,30 @@ extern "C" int fcs_gen_box(const fcs_gen* g, uint32_t x0, uint32_t y0, uint32_t 
   return 0;
 }
 
+extern "C" int fcs_gen_bricks(const fcs_gen* g, uint32_t b, uint32_t first, uint32_t count, uint32_t* out,
+                              int threads) {
+  if (!g || !out) return FCS_E_INVALID;
+  const uint32_t L = fcs_log2u(b);
+  if (b < 2 || (1u << L) != b || L > 6 || g->p.size % b) return FCS_E_INVALID;
+  const uint32_t B = g->p.size / b;
+  if (B > 1024 || (uint64_t)first + count > (uint64_t)B * B * B) return FCS_E_INVALID;
+  const uint64_t nvox = 1ull << (3 * L);
+  // one task per (brick, z-slice of the brick): Morton order within a brick
+  // is filled through the per-axis dilated coordinates
+  parallel_for((size_t)count * b, resolve_threads(threads), [&](size_t task) {
+    const uint32_t k = (uint32_t)(task / b), dz = (uint32_t)(task % b);
+    const uint32_t id = first + k;
+    const uint32_t bx = fcs_unmorton_x(id), by = fcs_unmorton_y(id), bz = fcs_unmorton_z(id);
+    uint32_t* o = out + nvox * k;
+    const uint32_t mz = fcs_part1by2(dz) << 2;
+    for (uint32_t dy = 0; dy < b; ++dy) {
+      const uint32_t myz = mz | (fcs_part1by2(dy) << 1);
+      for (uint32_t dx = 0; dx < b; ++dx) o[myz | fcs_part1by2(dx)] = g->at(bx * b + dx, by * b + dy, bz * b + dz);
+    }
+  });
+  return 0;
+}
+
 extern "C" double fcs_gen_background_fraction(const fcs_gen* g) {
   if (!g) return -1.0;
   size_t zero = 0;

@@ -160,7 +160,35 @@ GraphEntry* find_graph(fcs_view* v, const fcs_volume_desc& d, uint32_t w, uint32
   return nullptr;
 }
 
+// Most graphs a view keeps; the oldest is destroyed to make room.
+constexpr size_t kMaxViewGraphs = 64;
+
+void destroy_graph(GraphEntry& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+}
+
+// Called with the view's stream idle (its last frame waited for): graphs of
+// the same volume and frame size whose cache or shading epoch is stale can
+// never be replayed again and are destroyed; beyond kMaxViewGraphs the oldest
+// goes too.
+void evict_graphs(fcs_view* v, const fcs_volume_desc& d, uint32_t w, uint32_t h) {
+  auto stale = [&](const GraphEntry& g) { return g.serial == d.serial && g.w == w && g.h == h; };
+  for (auto& g : v->graphs)
+    if (stale(g)) destroy_graph(g);
+  v->graphs.erase(std::remove_if(v->graphs.begin(), v->graphs.end(), [](const GraphEntry& g) { return !g.exec; }),
+                  v->graphs.end());
+  while (v->graphs.size() >= kMaxViewGraphs) {
+    destroy_graph(v->graphs.front());
+    v->graphs.erase(v->graphs.begin());
+  }
+}
+
 int capture_graph(fcs_view* v, const fcs_volume_desc& d, uint32_t w, uint32_t h, GraphEntry** out) {
+  VIEW_TRY(v, hipStreamSynchronize(v->stream));
+  evict_graphs(v, d, w, h);
   VIEW_TRY(v, hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
   const int rc = enqueue_frame(v, d, false, w, h);
   hipGraph_t g = nullptr;

@@ -191,6 +191,7 @@ _sig("fcs_gen_create", ctypes.c_int, ctypes.POINTER(GenParams), ctypes.POINTER(c
 _sig("fcs_gen_destroy", None, c_vp)
 _sig("fcs_gen_box", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
      ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_int)
+_sig("fcs_gen_bricks", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_int)
 _sig("fcs_gen_background_fraction", ctypes.c_double, c_vp)
 _sig("fcs_encode_volume", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
      ctypes.POINTER(EncodeOpts), ctypes.POINTER(c_u8p), ctypes.POINTER(ctypes.c_size_t))

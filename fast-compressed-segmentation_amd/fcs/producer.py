@@ -33,6 +33,14 @@ class Generator:
         check(lib.fcs_gen_box(self.h, x0, y0, z0, nx, ny, nz, out.ctypes.data_as(c_u32p), threads))
         return out.reshape(nz, ny, nx)
 
+    def bricks(self, b, first, count, threads=0):
+        """Source labels of bricks first..first+count-1 (Morton IDs), each
+        brick's b^3 labels in Morton order: shape (count, b^3)."""
+        out = np.zeros((int(count), int(b) ** 3), np.uint32)
+        check(lib.fcs_gen_bricks(self.h, int(b), int(first), int(count), out.ctypes.data_as(c_u32p),
+                                 _threads(threads)))
+        return out
+
     def volume(self, threads=0):
         n = self.size
         return self.box(0, 0, 0, n, n, n, threads)

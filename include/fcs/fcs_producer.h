@@ -60,6 +60,13 @@ void fcs_gen_destroy(fcs_gen* g);
  * out[x + nx*(y + ny*z)]. */
 int fcs_gen_box(const fcs_gen* g, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t nx,
                 uint32_t ny, uint32_t nz, uint32_t* out, int threads);
+/* Source labels of `count` consecutive bricks of edge b, starting at brick
+ * Morton ID `first` (the order bricks are loaded in, datasetLoader.js:143),
+ * each brick's b^3 labels in Morton order (the decoder's finest-LOD layout,
+ * decode.wgsl:239-245), brick after brick: out[b^3 * k + morton(dx, dy, dz)].
+ * The finest LOD of a decoded brick equals this (closed form). */
+int fcs_gen_bricks(const fcs_gen* g, uint32_t b, uint32_t first, uint32_t count, uint32_t* out,
+                   int threads);
 /* Fraction of background (label 0) voxels, estimated on a 64^3 lattice. */
 double fcs_gen_background_fraction(const fcs_gen* g);
 

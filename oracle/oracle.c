@@ -814,3 +814,78 @@ uint64_t or_decode_all_pyramid(const or_dataset* d, uint64_t* out, int threads, 
   if (status_or) *status_or = P.status;
   return (uint64_t)d->num_bricks * (((1ull << (3 * (L + 1))) - 1) / 7);
 }
+
+/* ------------------------------------------- threaded request decode ---- */
+/* The checker for batched device launches: request k is (brick[k], lod[k]),
+ * its 8^lod labels written at out + offset[k] in the label width of `out32`
+ * (uint32 when non-NULL, else uint64 into out64); status words per request.
+ * Same restatement as or_decode_dataset_brick, run over a thread pool. */
+typedef struct {
+  const or_dataset* d;
+  const uint32_t* brick;
+  const uint8_t* lod;
+  const uint64_t* offset;
+  uint32_t* out32;
+  uint64_t* out64;
+  uint32_t* status;
+  uint32_t n, next;
+  pthread_mutex_t mu;
+} req_pool_t;
+
+static void* req_worker(void* arg) {
+  req_pool_t* P = (req_pool_t*)arg;
+  uint8_t* nib = NULL;
+  size_t cap = 0;
+  uint64_t* tmp = NULL;
+  size_t tcap = 0;
+  for (;;) {
+    pthread_mutex_lock(&P->mu);
+    const uint32_t k = P->next;
+    P->next += 64;
+    pthread_mutex_unlock(&P->mu);
+    if (k >= P->n) break;
+    const uint32_t end = k + 64 < P->n ? k + 64 : P->n;
+    for (uint32_t q = k; q < end; ++q) {
+      const uint32_t i = P->brick[q], t = P->lod[q];
+      if (i >= P->d->num_bricks) { P->status[q] = ST_OVERRUN; continue; }
+      const or_brick* b = &P->d->bricks[i];
+      const size_t need = (b->nsym + 2) / 2 + 1;
+      if (need > cap) { free(nib); nib = (uint8_t*)malloc(need); cap = need; }
+      const size_t nl = (size_t)1 << (3 * t);
+      if (nl > tcap) { free(tmp); tmp = (uint64_t*)malloc(nl * 8); tcap = nl; }
+      uint32_t st1 = 0, st2 = 0;
+      const size_t nb = or_rans_decode(P->d, i, nib, &st1);
+      uint64_t* dst = P->out32 ? tmp : P->out64 + P->offset[q];
+      or_decode_brick(nib, nb, P->d->palette + b->pal_off, b->P, t, dst, &st2);
+      if (P->out32)
+        for (size_t j = 0; j < nl; ++j) P->out32[P->offset[q] + j] = (uint32_t)tmp[j];
+      P->status[q] = st1 | st2;
+    }
+  }
+  free(nib);
+  free(tmp);
+  return NULL;
+}
+
+int or_decode_requests(const or_dataset* d, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                       const uint64_t* offset, uint32_t* out32, uint64_t* out64, int threads,
+                       uint32_t* status) {
+  req_pool_t P;
+  P.d = d;
+  P.brick = brick;
+  P.lod = lod;
+  P.offset = offset;
+  P.out32 = out32;
+  P.out64 = out64;
+  P.status = status;
+  P.n = n;
+  P.next = 0;
+  pthread_mutex_init(&P.mu, NULL);
+  if (threads < 1) threads = 1;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, req_worker, &P);
+  for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  pthread_mutex_destroy(&P.mu);
+  return 0;
+}

@@ -102,6 +102,13 @@ uint32_t or_brick_lod(double cx, double cy, double cz, double brick_size, double
 uint64_t or_decode_all_pyramid(const or_dataset* d, uint64_t* out, int threads,
                                uint32_t* status_or);
 
+/* Threaded decode of n (brick, LOD) requests, request k's 8^lod[k] labels
+ * at offset[k] of out32 (uint32, when non-NULL) or out64; status[k] gets
+ * its status bits. The checker of batched device launches. */
+int or_decode_requests(const or_dataset* d, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                       const uint64_t* offset, uint32_t* out32, uint64_t* out64, int threads,
+                       uint32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,8 @@ def lib():
         L.or_brick_lod.restype = ctypes.c_uint32
         L.or_decode_all_pyramid.argtypes = [ctypes.POINTER(OrDataset), c_u64p, ctypes.c_int, c_u32p]
         L.or_decode_all_pyramid.restype = ctypes.c_uint64
+        L.or_decode_requests.argtypes = [ctypes.POINTER(OrDataset), c_u32p, c_u8p, ctypes.c_uint32, c_u64p, c_u32p,
+                                         c_u64p, ctypes.c_int, c_u32p]
         _lib = L
     return _lib
 
@@ -120,6 +122,26 @@ class Dataset:
         lib().or_decode_all_pyramid(ctypes.byref(self.d), out.ctypes.data_as(c_u64p), threads or (os.cpu_count() or 1),
                                     ctypes.byref(st))
         return out.reshape(self.num_bricks, per), st.value
+
+
+    def decode_requests(self, bricks, lods, label_dtype=np.uint64, threads=0):
+        """or_decode_requests: (labels of every request concatenated in
+        request order, start offset of each request, status words)."""
+        b = np.ascontiguousarray(bricks, dtype=np.uint32)
+        t = np.ascontiguousarray(lods, dtype=np.uint8)
+        sizes = np.left_shift(np.uint64(1), np.uint64(3) * t.astype(np.uint64))
+        off = np.zeros(b.size, np.uint64)
+        if b.size:
+            off[1:] = np.cumsum(sizes)[:-1]
+        total = int(sizes.sum()) if b.size else 0
+        out = np.zeros(total, label_dtype)
+        st = np.zeros(b.size, np.uint32)
+        wide = np.dtype(label_dtype) == np.uint64
+        lib().or_decode_requests(ctypes.byref(self.d), b.ctypes.data_as(c_u32p), t.ctypes.data_as(c_u8p), b.size,
+                                 off.ctypes.data_as(c_u64p), None if wide else out.ctypes.data_as(c_u32p),
+                                 out.ctypes.data_as(c_u64p) if wide else None, threads or (os.cpu_count() or 1),
+                                 st.ctypes.data_as(c_u32p))
+        return out, off, st
 
 
 def decode_nibbles(nib_bytes, palette, t):
@@ -232,6 +254,8 @@ def ref():
         R.ref_brick_id.argtypes = [ctypes.c_uint32]
         R.ref_brick_id.restype = ctypes.c_uint32
         R.ref_decode_finest.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_u64p]
+        R.ref_decode_finest_batch.argtypes = [c_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, c_u64p, c_u32p]
+        R.ref_decode_finest_batch.restype = ctypes.c_int64
         _ref = R
     return _ref
 

@@ -11,9 +11,12 @@
 #include "brickEncoding.hpp"
 #include "fileIO.hpp"
 
+#include <atomic>
 #include <cstring>
 #include <exception>
 #include <string>
+#include <thread>
+#include <vector>
 
 namespace {
 
@@ -117,6 +120,36 @@ int ref_decode_finest(uint32_t idx, uint32_t b, uint64_t* out) {
   } catch (const std::exception&) {
     return -2;
   }
+}
+
+// CPU baseline: decodeRansStream + decodeBrick<b> at the finest LOD for the
+// bricks idx[0..n) (file order) on `threads` std::threads, labels discarded
+// after a checksum. Returns the number of bricks decoded; bricks the
+// reference rejects (it throws on a set root stop bit) are counted in
+// *rejected and skipped.
+int64_t ref_decode_finest_batch(const uint32_t* idx, uint32_t n, uint32_t b, int threads, uint64_t* checksum,
+                                uint32_t* rejected) {
+  if (!g_loaded || threads < 1) return -1;
+  std::atomic<uint32_t> next{0}, rej{0};
+  std::atomic<int64_t> done{0};
+  std::atomic<uint64_t> sum{0};
+  auto work = [&] {
+    std::vector<uint64_t> out((size_t)b * b * b);
+    uint64_t local = 0;
+    for (uint32_t k; (k = next.fetch_add(1)) < n;) {
+      const int r = ref_decode_finest(idx[k], b, out.data());
+      if (r < 0) { rej.fetch_add(1); continue; }
+      for (uint64_t x : out) local += x;
+      done.fetch_add(1);
+    }
+    sum.fetch_add(local);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  if (checksum) *checksum = sum.load();
+  if (rejected) *rejected = rej.load();
+  return done.load();
 }
 
 }  // extern "C"
