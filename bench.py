@@ -473,7 +473,11 @@ def cpu_baseline_config3(args, data, ds, nb, L):
                     chunk = nc[done2:done2 + 16 * threads]
                     list(ex.map(lambda i: ds.decode(int(i), L), chunk))
                     done2 += len(chunk)
-            res["port_on_nonconstant_bricks"] = done2 * per / (time.time() - t2) / 1e9
+            port_nc = done2 * per / (time.time() - t2) / 1e9
+            # the reference's own decoder is the baseline; the port beside it
+            ref_res = res.pop("reference")
+            ref_res["port"] = dict(res, value_on_nonconstant_bricks=port_nc)
+            return ref_res
     return res
 
 

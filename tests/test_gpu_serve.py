@@ -233,7 +233,8 @@ def test_view_u64_labels(gpu_ctx, manifest):
 @pytest.mark.parametrize("graph", [False, True])
 def test_shaded_view_frames_equal_render(gpu_ctx, vol256, graph):
     """Shaded views (the RGBA form) equal fcs_render's RGBA output; a lighting
-    change is picked up by graph replays (a new graph is captured)."""
+    change is picked up by graph replays (a new graph is captured and the
+    graph of the old lighting, which can never be replayed, is destroyed)."""
     vol, _ = vol256
     view = serve.View(gpu_ctx, 1280, 720, graph=graph, labels=True, checksum=True, rgba=True)
     cams = _cameras(vol)[:4]
@@ -249,8 +250,8 @@ def test_shaded_view_frames_equal_render(gpu_ctx, vol256, graph):
                 assert np.array_equal(view.labels(), ref.rgba), (angle, i)
                 assert np.array_equal(view.read_rgba(), ref.rgba)
                 assert np.array_equal(view.read().label, ref.label)
-        if graph:
-            assert view.graphs == 2
+            if graph:
+                assert view.graphs == 1
     finally:
         vol.set_lighting()
         view.close()
