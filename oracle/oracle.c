@@ -331,12 +331,14 @@ static float adv(int v[3], float tmax[3], const float tdelta[3], const int step[
   /* advanceDDA / advanceBrickDDA: ties prefer z, then y (strict <) */
   float t;
   if (tmax[0] < tmax[1] && tmax[0] < tmax[2]) { t = tmax[0]; tmax[0] += tdelta[0]; v[0] += step[0]; }
-  else if (tmax[1] < tmax[2]) { t = tmax[1]; tmax[1] += tdelta[1]; v[1] += step[1]; }
+  else if (OR_MUTANT == 7 ? tmax[1] <= tmax[2] : tmax[1] < tmax[2]) { t = tmax[1]; tmax[1] += tdelta[1]; v[1] += step[1]; }
   else { t = tmax[2]; tmax[2] += tdelta[2]; v[2] += step[2]; }
   return t;
 }
 
 static float bias(float t) { return fmaxf_(t * 1e-6f, 1e-5f); }
+/* the entry offset of initBrickDDA / initDDA (rayTrace.wgsl:522, 476) */
+static float entry_bias(float t) { return OR_MUTANT == 8 ? 0.0f : bias(t); }
 
 /* Scene of one frame; the compositing fields are used by or_render_rgba. */
 typedef struct {
@@ -568,7 +570,7 @@ static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uin
     int step[3], brick[3];
     float btmax[3], btdelta[3];
     {
-      const float ts = tEnter + bias(tEnter);
+      const float ts = tEnter + entry_bias(tEnter);
       for (int a = 0; a < 3; ++a) {
         const float pos = r.o[a] + r.d[a] * ts;
         brick[a] = (int)floorf(pos) / (int)b;
@@ -601,11 +603,11 @@ static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uin
         /* initDDA (rayTrace.wgsl:473-499) */
         int vox[3], vstep[3];
         float vtmax[3], vtdelta[3];
-        const float ts = tEnterBrick + bias(tEnterBrick);
+        const float ts = tEnterBrick + entry_bias(tEnterBrick);
         for (int a = 0; a < 3; ++a) {
           const float pos = r.o[a] + r.d[a] * ts;
           vox[a] = (int)floorf(pos);
-          vstep[a] = step[a];
+          vstep[a] = (OR_MUTANT == 9 && a == 0) ? -step[a] : step[a];
           const float nb = r.d[a] > 0.0f ? (float)(vox[a] + 1) : (float)vox[a];
           vtmax[a] = r.zero[a] ? INFINITY : (nb - r.o[a]) * r.inv[a];
           vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]);
@@ -625,7 +627,7 @@ static void march_pixel(const scene_t* sc, const or_camera* cam, uint32_t w, uin
             /* the reference keeps a unit voxel step here (rayTrace.wgsl:721-744)
              * while tDelta covers `stride` voxels, so coarse cells are sampled
              * out of place; the cell walk steps by the stride */
-            vstep[a] = step[a] * stride;
+            vstep[a] = vstep[a] * stride;
             vtdelta[a] = r.zero[a] ? INFINITY : fabsf(r.inv[a]) * (float)stride;
             const float lw = (float)bmin[a] + (float)vox[a];
             const float nb = r.d[a] > 0.0f ? lw + (float)stride : lw;

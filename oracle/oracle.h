@@ -13,9 +13,11 @@
  *                      pyramid dumped from the reference encoder; reference
  *                      decodeBrick<b> at the finest LOD; LOD 0 == palette[0].
  *   or_render          orthographic +z frames == brute-force column scan;
- *                      perspective frames: no independent pin beyond the
- *                      symmetric/axis-aligned cases in the tests: parity
- *                      partially pinned for perspective frames.
+ *                      perspective and orthographic frames == the float64
+ *                      exact traversal of dda_check.c (every pixel agrees or
+ *                      is a grazing tie within the FP32 rounding bound; the
+ *                      tie rule on exact ties); planted mistakes (tie order,
+ *                      entry epsilon, step sign) fail it.
  *   or_brick_lod       exact thresholds of camera.js:103-137 in tests.
  *   or_render_rgba     closed forms: a one-label volume seen along +z
  *                      composites to 1-(1-a)^n in the label's colour; a

@@ -12,6 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref.so")
+DDACHECK_SO = os.path.join(HERE, "libddacheck.so")
 
 c_vp = ctypes.c_void_p
 c_u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -275,3 +276,52 @@ def ref_encode(vol_zyx, brick_size, path, with_pyramid=False):
     with open(path, "rb") as f:
         data = f.read()
     return data, (pyr.reshape(n, -1) if pyr is not None else None)
+
+
+# ------------------------------------------------ float64 DDA check --
+class DcStats(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_uint64), ("hits", ctypes.c_uint64), ("agree", ctypes.c_uint64),
+                ("grazing", ctypes.c_uint64), ("unexplained", ctypes.c_uint64), ("max_t_err", ctypes.c_double)]
+
+
+_dc = None
+
+
+def dc():
+    global _dc
+    if _dc is None:
+        if not os.path.exists(DDACHECK_SO):
+            raise ImportError(f"{DDACHECK_SO} missing: run `make -C oracle`")
+        D = ctypes.CDLL(DDACHECK_SO)
+        D.dc_check_frame.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_u8p, c_u8p, ctypes.POINTER(c_u64p), c_u64p,
+                                     ctypes.POINTER(OrCamera), ctypes.c_uint32, ctypes.c_uint32, c_u64p, c_u64p,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_int, c_u64p, c_u64p, c_u8p,
+                                     ctypes.POINTER(DcStats)]
+        _dc = D
+    return _dc
+
+
+def check_frame(bricks_per_axis, brick_size, brick_empty, brick_lod, brick_labels, palette0, cam, w, h, label, voxel,
+                t_hit, threads=0):
+    """dc_check_frame: the float64 exact traversal against a first-hit frame.
+    Returns (stats dict, per-pixel class array, exact labels, exact voxels);
+    classes: 0 agree, 1 grazing tie, 2 unexplained."""
+    ptrs, keep = _ptrs(brick_labels)
+    c = _camera(cam)
+    empty = np.ascontiguousarray(brick_empty, dtype=np.uint8)
+    lods = np.ascontiguousarray(brick_lod, dtype=np.uint8)
+    p0 = np.ascontiguousarray(palette0, dtype=np.uint64)
+    gl = np.ascontiguousarray(label, dtype=np.uint64)
+    gv = np.ascontiguousarray(voxel, dtype=np.uint64)
+    gt = np.ascontiguousarray(t_hit, dtype=np.float32)
+    cls = np.zeros(w * h, np.uint8)
+    el = np.zeros(w * h, np.uint64)
+    ev = np.zeros(w * h, np.uint64)
+    st = DcStats()
+    dc().dc_check_frame(bricks_per_axis, brick_size, empty.ctypes.data_as(c_u8p), lods.ctypes.data_as(c_u8p), ptrs,
+                        p0.ctypes.data_as(c_u64p), ctypes.byref(c), w, h, gl.ctypes.data_as(c_u64p),
+                        gv.ctypes.data_as(c_u64p), gt.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                        threads or (os.cpu_count() or 1), el.ctypes.data_as(c_u64p), ev.ctypes.data_as(c_u64p),
+                        cls.ctypes.data_as(c_u8p), ctypes.byref(st))
+    stats = {f: getattr(st, f) for f, _ in DcStats._fields_}
+    return stats, cls, el, ev
