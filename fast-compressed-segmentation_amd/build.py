@@ -33,7 +33,7 @@ def build(verbose=True):
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src, kind in (("fcs_rans.hip", "hip"), ("fcs_ops.hip", "hip"), ("fcs_march.hip", "hip"),
-                      ("fcs_mlp.hip", "hip"), ("fcs_host.cpp", "host"), ("fcs_serve.cpp", "host"),
+                      ("fcs_mlp.hip", "hip"), ("fcs_scratch.hip", "hip"), ("fcs_host.cpp", "host"), ("fcs_serve.cpp", "host"),
                       ("fcs_producer.cpp", "host")):
         obj = os.path.join(objdir, src + ".o")
         if kind == "hip":

@@ -320,7 +320,7 @@ uint64_t account_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs) {
 constexpr size_t kOpsTuneMinRequests = 2048;
 
 // Decode `reqs` on the context stream and wait; statuses in caller order.
-int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
+int run_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& reqs, std::vector<uint32_t>& status,
                uint64_t* voxels_out) {
   fcs_ctx* c = v->ctx;
   status.assign(reqs.size(), 0);
@@ -337,7 +337,7 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     float t[2] = {0, 0};
     const int order[3] = {FCS_OPS_SPLIT_3WG, FCS_OPS_SPLIT_4WG, FCS_OPS_SPLIT_3WG};
     for (int k = 0; k < 3; ++k) {
-      if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true, order[k])) return rc;
+      if (int rc = launch_decode(v, ln, reqs, c->stream, true, order[k])) return rc;
       HIP_TRY(c, hipStreamSynchronize(c->stream));
       HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
       if (k) t[order[k] - 1] = ms;
@@ -348,11 +348,11 @@ int run_decode(fcs_volume* v, const std::vector<DecodeReq>& reqs, std::vector<ui
     ms = t[v->ops_tuned - 1];
     c->timing.ops_split = (uint32_t)v->ops_tuned;
   } else {
-    if (int rc = launch_decode(v, v->lane_main, reqs, c->stream, true)) return rc;
+    if (int rc = launch_decode(v, ln, reqs, c->stream, true)) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   }
-  finish_decode(v->lane_main, status);
+  finish_decode(ln, status);
   c->timing.decode_ms = ms;
   const uint64_t vox = account_decode(v, reqs);
   if (voxels_out) *voxels_out = vox;
@@ -551,6 +551,14 @@ int fcs_create(int hip_device, fcs_ctx** out) {
       hipEventCreateWithFlags(&c->main_mark, hipEventDisableTiming) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->aux) (void)hipStreamDestroy(c->aux);
+    delete c;
+    return FCS_E_HIP;
+  }
+  if (fcs_scratch_setup(hip_device) != 0) {
+    (void)hipEventDestroy(c->aux_done);
+    (void)hipEventDestroy(c->main_mark);
+    (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->aux);
     delete c;
     return FCS_E_HIP;
   }
@@ -1000,12 +1008,18 @@ int fcs_cache_reserve_direct(fcs_volume* v) {
   return 0;
 }
 
-static int request_bricks_body(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
-                       uint64_t* out_off, uint32_t* out_status) {
-  if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
+// The requests of one call decoded through lane `ln`. With `no_alloc` the
+// call allocates no device memory: the cache pool and the rANS stage must
+// already exist (the lane's buffers are the caller's).
+static int request_bricks_impl(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                               uint64_t* out_off, uint32_t* out_status, DecodeLane& ln, bool no_alloc) {
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   if (int rc = settle_prefetch(v)) return rc;
+  if (no_alloc) {
+    if (!v->rans_ready) return fail(c, FCS_E_INVALID, "fcs_decode_rans has not run");
+    if (!v->d_cache) return fail(c, FCS_E_INVALID, "no brick cache pool reserved");
+  }
   if (!v->rans_ready) {
     int rc = fcs_decode_rans(v, nullptr);
     if (rc && rc != FCS_E_CORRUPT) return rc;
@@ -1057,7 +1071,7 @@ static int request_bricks_body(fcs_volume* v, const uint32_t* brick, const uint8
     if (out_off) out_off[i] = off;
   }
   std::vector<uint32_t> rs;
-  const int rc_launch = run_decode(v, reqs, rs, nullptr);
+  const int rc_launch = run_decode(v, ln, reqs, rs, nullptr);
   cache_free_deferred(v, deferred);
   if (rc_launch) return rc_launch;
   bool bad = false;
@@ -1078,7 +1092,25 @@ static int request_bricks_body(fcs_volume* v, const uint32_t* brick, const uint8
 
 int fcs_request_bricks(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
                        uint64_t* out_off, uint32_t* out_status) {
-  return fcs_guard(v ? v->ctx : nullptr, [&] { return request_bricks_body(v, brick, lod, n, out_off, out_status); });
+  if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
+  return fcs_guard(v->ctx, [&] { return request_bricks_impl(v, brick, lod, n, out_off, out_status, v->lane_main, false); });
+}
+
+int fcs_request_bricks_static(fcs_volume* v, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                              uint64_t* out_off, uint32_t* out_status) {
+  if (!v || !v->ctx || (n && (!brick || !lod))) return FCS_E_INVALID;
+  fcs_ctx* c = v->ctx;
+  if (n > FCS_STATIC_MAX_REQUESTS)
+    return fail(c, FCS_E_INVALID, "more than FCS_STATIC_MAX_REQUESTS requests in one call");
+  return fcs_guard(c, [&] {
+    // the lane's device buffers are the GPU's fixed scratch: ensure_req sees
+    // them as already large enough and allocates nothing
+    DecodeLane ln;
+    if (int rc = fcs_scratch_lane(c->device, &ln.d_counter, &ln.d_req, &ln.d_status))
+      return fail(c, rc, "no fixed scratch on this device");
+    ln.cap = FCS_STATIC_MAX_REQUESTS;
+    return request_bricks_impl(v, brick, lod, n, out_off, out_status, ln, true);
+  });
 }
 
 // Cache slots and requests for every brick at `lod` (brick i at slot i).
@@ -1115,7 +1147,7 @@ static int decode_all_body(fcs_volume* v, uint32_t lod, int pyramid) {
   std::vector<DecodeReq> reqs;
   if (int rc = full_requests(v, lod, pyramid, reqs)) return rc;
   std::vector<uint32_t> rs;
-  if (int rc = run_decode(v, reqs, rs, nullptr)) return rc;
+  if (int rc = run_decode(v, v->lane_main, reqs, rs, nullptr)) return rc;
   uint32_t bad = 0, first = ~0u, any = 0;
   for (uint32_t i = 0; i < v->nb; ++i)
     if (rs[i]) {

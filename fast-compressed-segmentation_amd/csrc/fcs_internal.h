@@ -11,6 +11,15 @@
 
 struct fcs_volume;
 struct fcs_mlp;
+struct DecodeReq;
+
+// Fixed device scratch (fcs_scratch.hip) of fcs_request_bricks_static: its
+// address on `device` is looked up once (fcs_create, device current);
+// fcs_scratch_lane hands out the claim counter, request list and status words
+// of FCS_STATIC_MAX_REQUESTS requests inside it.
+int fcs_scratch_setup(int device);
+int fcs_scratch_lane(int device, uint32_t** counter, DecodeReq** req, uint32_t** status);
+size_t fcs_scratch_bytes();
 
 // One HIP device and its streams (fcs_create). `stream` runs requests and
 // frames; `aux` runs work issued ahead of them (prefetch decodes). The context

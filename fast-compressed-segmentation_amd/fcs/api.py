@@ -175,6 +175,19 @@ class Volume:
             check(rc, self.ctx.h)
         return off, st
 
+    def request_bricks_static(self, bricks, lods, raise_on_error=True):
+        """fcs_request_bricks_static: the same as request_bricks with no device
+        allocation (the cache pool must be reserved and decode_rans done)."""
+        b = np.ascontiguousarray(bricks, dtype=np.uint32)
+        l = np.ascontiguousarray(lods, dtype=np.uint8)
+        off = np.zeros(b.size, np.uint64)
+        st = np.zeros(b.size, np.uint32)
+        rc = lib.fcs_request_bricks_static(self.h, _ptr(b, c_u32p), _ptr(l, c_u8p), b.size, _ptr(off, c_u64p),
+                                           _ptr(st, c_u32p))
+        if rc < 0 and (raise_on_error or rc != -6):
+            check(rc, self.ctx.h)
+        return off, st
+
     def decode_all(self, lod=255, pyramid=True):
         check(lib.fcs_decode_all(self.h, int(lod), 1 if pyramid else 0), self.ctx.h)
 

@@ -196,6 +196,22 @@ int fcs_cache_clear(fcs_volume* vol);
 int fcs_request_bricks(fcs_volume* vol, const uint32_t* brick, const uint8_t* lod, uint32_t n,
                        uint64_t* out_cache_offset, uint32_t* out_status);
 
+/* No reference counterpart: fcs_request_bricks for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * hosts that forbid allocation). Same requests, offsets, statuses and labels,
+ * but the op decoder's request list, status words and claim counter live in a
+ * fixed device array of the library (one per GPU, held until the process
+ * ends) instead of buffers it allocates, and the call performs no device
+ * allocation or free: the brick cache pool must already be reserved
+ * (fcs_cache_reserve / fcs_cache_reserve_direct) and the rANS stage must have
+ * run (fcs_decode_rans), else FCS_E_INVALID. At most FCS_STATIC_MAX_REQUESTS
+ * requests per call (more: FCS_E_INVALID). Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other threads, other contexts) race
+ * and must be serialised by the caller. */
+#define FCS_STATIC_MAX_REQUESTS 262144u
+int fcs_request_bricks_static(fcs_volume* vol, const uint32_t* brick, const uint8_t* lod, uint32_t n,
+                              uint64_t* out_cache_offset, uint32_t* out_status);
+
 /* Decode every brick at LOD `lod` (lod > levels: every brick at its finest LOD).
  * pyramid != 0 writes all levels 0..lod of each brick (coarse-first layout). */
 int fcs_decode_all(fcs_volume* vol, uint32_t lod, int pyramid);
