@@ -999,6 +999,20 @@ int fcs_cache_clear(fcs_volume* v) {
   return 0;
 }
 
+int64_t fcs_cache_resident(const fcs_volume* v, uint32_t* bricks, uint8_t* lods, uint64_t cap) {
+  if (!v || !v->ctx) return FCS_E_INVALID;
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    if (v->res_lod[i] < 0) continue;
+    if (n < cap) {
+      if (bricks) bricks[n] = i;
+      if (lods) lods[n] = (uint8_t)v->res_lod[i];
+    }
+    ++n;
+  }
+  return (int64_t)n;
+}
+
 int fcs_cache_reserve_direct(fcs_volume* v) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   const uint64_t finest = 1ull << (3 * v->L);

@@ -168,6 +168,7 @@ _sig("fcs_cache_reserve", ctypes.c_int, c_vp, ctypes.c_uint64)
 _sig("fcs_cache_clear", ctypes.c_int, c_vp)
 _sig("fcs_cache_reserve_direct", ctypes.c_int, c_vp)
 _sig("fcs_request_bricks", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32, c_u64p, c_u32p)
+_sig("fcs_cache_resident", ctypes.c_int64, c_vp, c_u32p, c_u8p, ctypes.c_uint64)
 _sig("fcs_request_bricks_static", ctypes.c_int, c_vp, c_u32p, c_u8p, ctypes.c_uint32, c_u64p, c_u32p)
 STATIC_MAX_REQUESTS = 262144
 _sig("fcs_decode_all", ctypes.c_int, c_vp, ctypes.c_uint32, ctypes.c_int)

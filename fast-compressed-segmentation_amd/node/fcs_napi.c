@@ -167,7 +167,7 @@ static napi_value js_load(napi_env env, napi_callback_info info) {
   return obj;
 }
 
-/* decodeAllRansBricks(volume) -> Uint32Array status per brick (fcs_decode_rans) */
+/* decodeRans(volume) -> Uint32Array status per brick (fcs_decode_rans) */
 static napi_value js_decode_rans(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -179,6 +179,80 @@ static napi_value js_decode_rans(napi_env env, napi_callback_info info) {
   int rc = fcs_decode_rans(vb->vol, (uint32_t*)st);
   if (rc < 0 && rc != FCS_E_CORRUPT) return throw_fcs(env, rc, vb->ctx, "fcs_decode_rans");
   return arr;
+}
+
+/* readNibbles(volume, brick) -> Uint8Array: the brick's nibble stream after
+ * the device rANS stage, two nibbles per byte, the first in the high half
+ * (one element of decodeAllRansBricks' result, decode.js:17-26) */
+static napi_value nibbles_of(napi_env env, vol_box* vb, uint32_t brick) {
+  const int64_t n = fcs_read_nibbles(vb->vol, brick, NULL, 0);
+  if (n < 0) return throw_fcs(env, (int)n, vb->ctx, "fcs_read_nibbles");
+  void* p;
+  napi_value arr = new_typed(env, napi_uint8_array, (size_t)n, 1, &p);
+  if (!arr) return NULL;
+  if (n > 0) {
+    const int64_t rc = fcs_read_nibbles(vb->vol, brick, (uint8_t*)p, (uint64_t)n);
+    if (rc < 0) return throw_fcs(env, (int)rc, vb->ctx, "fcs_read_nibbles");
+  }
+  return arr;
+}
+
+static napi_value js_read_nibbles(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  uint32_t brick;
+  if (!vb || !get_u32(env, argv[1], &brick)) return NULL;
+  return nibbles_of(env, vb, brick);
+}
+
+/* allNibbles(volume) -> Array<Uint8Array>, one nibble stream per brick in
+ * brick (Morton ID) order */
+static napi_value js_all_nibbles(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  napi_value arr;
+  if (napi_create_array_with_length(env, vb->info.num_bricks, &arr) != napi_ok) return NULL;
+  for (uint32_t b = 0; b < vb->info.num_bricks; ++b) {
+    napi_value e = nibbles_of(env, vb, b);
+    if (!e) return NULL;
+    napi_set_element(env, arr, b, e);
+  }
+  return arr;
+}
+
+/* cacheClear(volume): drop every resident brick (fcs_cache_clear) */
+static napi_value js_cache_clear(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  int rc = fcs_cache_clear(vb->vol);
+  if (rc < 0) return throw_fcs(env, rc, vb->ctx, "fcs_cache_clear");
+  return NULL;
+}
+
+/* cacheResident(volume) -> { bricks: Uint32Array, lods: Uint8Array } (fcs_cache_resident) */
+static napi_value js_cache_resident(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  vol_box* vb = unwrap_vol(env, argv[0]);
+  if (!vb) return NULL;
+  int64_t n = fcs_cache_resident(vb->vol, NULL, NULL, 0);
+  if (n < 0) return throw_fcs(env, (int)n, vb->ctx, "fcs_cache_resident");
+  void *bp, *lp;
+  napi_value jb = new_typed(env, napi_uint32_array, (size_t)n, 4, &bp);
+  napi_value jl = new_typed(env, napi_uint8_array, (size_t)n, 1, &lp);
+  if (!jb || !jl) return NULL;
+  n = fcs_cache_resident(vb->vol, (uint32_t*)bp, (uint8_t*)lp, (uint64_t)n);
+  if (n < 0) return throw_fcs(env, (int)n, vb->ctx, "fcs_cache_resident");
+  napi_value obj;
+  napi_create_object(env, &obj);
+  napi_set_named_property(env, obj, "bricks", jb);
+  napi_set_named_property(env, obj, "lods", jl);
+  return obj;
 }
 
 /* requestBricks(volume, Uint32Array bricks, Uint8Array lods)
@@ -477,7 +551,11 @@ static napi_value init(napi_env env, napi_value exports) {
       {"deviceCount", NULL, js_device_count, NULL, NULL, NULL, napi_default, NULL},
       {"createContext", NULL, js_create_context, NULL, NULL, NULL, napi_default, NULL},
       {"loadCsdb", NULL, js_load, NULL, NULL, NULL, napi_default, NULL},
-      {"decodeAllRansBricks", NULL, js_decode_rans, NULL, NULL, NULL, napi_default, NULL},
+      {"decodeRans", NULL, js_decode_rans, NULL, NULL, NULL, napi_default, NULL},
+      {"readNibbles", NULL, js_read_nibbles, NULL, NULL, NULL, napi_default, NULL},
+      {"allNibbles", NULL, js_all_nibbles, NULL, NULL, NULL, napi_default, NULL},
+      {"cacheClear", NULL, js_cache_clear, NULL, NULL, NULL, napi_default, NULL},
+      {"cacheResident", NULL, js_cache_resident, NULL, NULL, NULL, napi_default, NULL},
       {"requestBricks", NULL, js_request, NULL, NULL, NULL, napi_default, NULL},
       {"decodeAll", NULL, js_decode_all, NULL, NULL, NULL, napi_default, NULL},
       {"decodeVolume", NULL, js_decode_volume, NULL, NULL, NULL, napi_default, NULL},

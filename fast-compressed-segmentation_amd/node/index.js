@@ -25,9 +25,57 @@ function loadDatasetFromArrayBuffer(buffer, brickSizeOverride = 0, ctx = null) {
   return addon.loadCsdb(ctx || context(), bytes, brickSizeOverride);
 }
 
-/** Device rANS stage for every brick; returns per-brick status words. */
-function decodeAllRansBricks(dataset) {
-  return addon.decodeAllRansBricks(dataset);
+/** decodeAllRansBricks (web/scripts/decode.js:17-26): resolves to one
+ * Uint8Array per brick (brick order), its rANS-decoded nibble stream with two
+ * nibbles per byte, the first in the high half -- what web/app.js:75-82
+ * stores as each brick's encodedData. The rANS stage runs on the device and
+ * its output stays resident there for decoding; the arrays are host copies.
+ * Per-brick status words of the stage: decodeRansStatus / dataset.ransStatus.
+ * Rejects with "rANS decode failed" when a brick's stream is corrupt. */
+async function decodeAllRansBricks(dataset) {
+  const status = decodeRansStatus(dataset);
+  const bad = status.findIndex((s) => s !== 0);
+  if (bad >= 0) throw new Error(`rANS decode failed (brick ${bad}, status 0x${status[bad].toString(16)})`);
+  return addon.allNibbles(dataset);
+}
+
+/** The device rANS stage for every brick (fcs_decode_rans): per-brick
+ * status words (0 = ok), also kept as dataset.ransStatus. */
+function decodeRansStatus(dataset) {
+  const status = addon.decodeRans(dataset);
+  dataset.ransStatus = status;
+  return status;
+}
+
+/** One brick's nibble stream (an element of decodeAllRansBricks' result). */
+function readNibbles(dataset, brick) {
+  return addon.readNibbles(dataset, brick);
+}
+
+/** Decode again every brick that is resident, at the LOD it holds (its
+ * finest level for a pyramid), after dropping the whole cache. Returns the
+ * number of bricks decoded. */
+function redecodeResident(dataset) {
+  const res = addon.cacheResident(dataset);
+  addon.cacheClear(dataset);
+  if (res.bricks.length) addon.requestBricks(dataset, res.bricks, res.lods);
+  return res.bricks.length;
+}
+
+/** forceFullBrickRefresh (web/scripts/renderLoop.js:502-559): reset the
+ * brick cache and immediately decode again the bricks that were resident.
+ * The reference needs this after a visibility change because its cache holds
+ * colours; this cache holds labels and colours are applied while marching,
+ * so the images are already current -- the call keeps the reference's
+ * effect on the cache. */
+function forceFullBrickRefresh(dataset) {
+  return redecodeResident(dataset);
+}
+
+/** reDecodeCachedBricks (web/scripts/renderLoop.js:565-595): decode every
+ * resident brick again. */
+function reDecodeCachedBricks(dataset) {
+  return redecodeResident(dataset);
 }
 
 // Per-dataset label visibility state: applyLabelVisibility and
@@ -74,6 +122,12 @@ module.exports = {
   createContext: addon.createContext,
   loadDatasetFromArrayBuffer,
   decodeAllRansBricks,
+  decodeRansStatus,
+  readNibbles,
+  forceFullBrickRefresh,
+  reDecodeCachedBricks,
+  cacheResident: addon.cacheResident,
+  cacheClear: addon.cacheClear,
   requestBricks: addon.requestBricks,
   decodeAll: addon.decodeAll,
   decodeVolume: addon.decodeVolume,

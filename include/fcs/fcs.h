@@ -185,6 +185,12 @@ int fcs_decode_rans(fcs_volume* vol, uint32_t* out_status);
 int fcs_cache_reserve(fcs_volume* vol, uint64_t voxels);
 /* Drop every resident brick (pool kept). */
 int fcs_cache_clear(fcs_volume* vol);
+/* The resident set, the brick cache's residency map (cacheManager.js:52-78,
+ * brickCache.residency read by forceFullBrickRefresh / reDecodeCachedBricks,
+ * renderLoop.js:502-595): resident bricks in ascending index order with the
+ * LOD each holds (its finest level for a pyramid); writes min(cap, n) entries
+ * (either pointer nullable) and returns n. */
+int64_t fcs_cache_resident(const fcs_volume* vol, uint32_t* bricks, uint8_t* lods, uint64_t cap);
 
 /* Decode n (brick, LOD) requests into the brick cache (one batched launch).
  * Bricks already resident at that LOD are not decoded again.

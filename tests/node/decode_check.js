@@ -9,9 +9,19 @@ const fcs = require(path.join(__dirname, '..', '..', 'fast-compressed-segmentati
 const [csdbPath, outDir, brickSize] = process.argv.slice(2);
 const buf = fs.readFileSync(csdbPath);
 const ds = fcs.loadDatasetFromArrayBuffer(new Uint8Array(buf.buffer, buf.byteOffset, buf.length), Number(brickSize));
-const st = fcs.decodeAllRansBricks(ds);
-fs.writeFileSync(path.join(outDir, 'rans_status.bin'), Buffer.from(st.buffer));
 const h = ds.header;
+
+async function main() {
+// app.js:75-82: decodeAllRansBricks resolves to one nibble stream per brick
+const nibbles = await fcs.decodeAllRansBricks(ds);
+const st = ds.ransStatus;
+fs.writeFileSync(path.join(outDir, 'rans_status.bin'), Buffer.from(st.buffer));
+fs.writeFileSync(path.join(outDir, 'nibbles.bin'), Buffer.concat(nibbles.map((a) => Buffer.from(a.buffer))));
+fs.writeFileSync(path.join(outDir, 'nibble_lengths.bin'),
+                 Buffer.from(Uint32Array.from(nibbles.map((a) => a.length)).buffer));
+const nibOk = nibbles.every((a) => a instanceof Uint8Array);
+const one = fcs.readNibbles(ds, 7);
+const readNibblesMatches = one.length === nibbles[7].length && one.every((x, i) => x === nibbles[7][i]);
 // every brick at every LOD through readBrick
 const parts = [];
 for (let b = 0; b < h.numBricks; b++) {
@@ -43,7 +53,22 @@ fcs.applyLabelHover(ds, labels[0]);
 const hiddenAll = fcs.render(ds, cam, 256, 256, lods, true).rgba.every((x) => x === 0);
 fcs.applyLabelVisibility(ds, []);
 fcs.applyLabelHover(ds, null);
+// forceFullBrickRefresh / reDecodeCachedBricks: the resident set decoded again
+const before = fcs.cacheResident(ds);
+const probe = fcs.readBrick(ds, 5, h.levels);
+const refreshed = fcs.forceFullBrickRefresh(ds);
+const again = fcs.reDecodeCachedBricks(ds);
+const after = fcs.cacheResident(ds);
+const probe2 = fcs.readBrick(ds, 5, h.levels);
+const sameLabels = refreshed === again && after.bricks.length === before.bricks.length &&
+                   probe.every((x, i) => x === probe2[i]);
 let threw = false;
 try { fcs.loadDatasetFromArrayBuffer(new Uint8Array(64)); } catch (e) { threw = /Invalid file format/.test(e.message); }
 console.log(JSON.stringify({ header: h, badMagicThrows: threw, marchMs: f.marchMs, numLabels: labels.length,
-                              hiddenAllTransparent: hiddenAll }));
+                              hiddenAllTransparent: hiddenAll, nibbleArrays: nibbles.length,
+                              nibblesAreUint8Arrays: nibOk, readNibblesMatches,
+                              residentBefore: before.bricks.length, refreshResident: refreshed,
+                              refreshSameLabels: sameLabels }));
+}
+
+main().catch((e) => { console.error(e); process.exit(1); });

@@ -16,7 +16,8 @@ NODE = shutil.which("node")
 ADDON = os.path.join(ROOT, "fast-compressed-segmentation_amd", "node", "fcs.node")
 pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the addon missing")
 
-API = ["deviceCount", "createContext", "loadDatasetFromArrayBuffer", "decodeAllRansBricks", "requestBricks",
+API = ["deviceCount", "createContext", "loadDatasetFromArrayBuffer", "decodeAllRansBricks", "decodeRansStatus",
+       "readNibbles", "forceFullBrickRefresh", "reDecodeCachedBricks", "cacheResident", "cacheClear", "requestBricks",
        "decodeAll", "decodeVolume", "readBrick", "orbitCamera", "render", "updateLightingOptions", "applyLabelVisibility",
        "applyLabelHover", "getDatasetLabels", "transferFunction"]
 
@@ -42,11 +43,25 @@ def test_node_path_bit_exact(tmp_path):
     assert meta["badMagicThrows"] is True
     st = np.fromfile(tmp_path / "rans_status.bin", np.uint32)
     assert st.shape == (64,) and np.all(st == 0)
+    # decodeAllRansBricks resolved to one nibble Uint8Array per brick, equal to
+    # the oracle's decodeRansStream restatement (decode.js:17-26)
+    ds = O.Dataset(golden_bytes("cfg1_b32"), 32)
+    nib = np.fromfile(tmp_path / "nibbles.bin", np.uint8)
+    lens = np.fromfile(tmp_path / "nibble_lengths.bin", np.uint32)
+    assert lens.shape == (64,) and meta["nibbleArrays"] == 64 and meta["nibblesAreUint8Arrays"] is True
+    pos = 0
+    for b in range(64):
+        ref, rst = ds.rans(b)
+        assert rst == 0 and int(lens[b]) == ref.size
+        assert np.array_equal(nib[pos:pos + ref.size], ref), b
+        pos += ref.size
+    assert meta["readNibblesMatches"] is True
+    # forceFullBrickRefresh / reDecodeCachedBricks decode the resident set again
+    assert meta["refreshResident"] == meta["residentBefore"] > 0 and meta["refreshSameLabels"] is True
     vst = np.fromfile(tmp_path / "volume_status.bin", np.uint32)
     assert vst.shape == (64,) and np.all(vst == 0)
     assert np.array_equal(np.fromfile(tmp_path / "bricks_after_volume.bin", np.uint32),
                           np.fromfile(tmp_path / "bricks.bin", np.uint32))
-    ds = O.Dataset(golden_bytes("cfg1_b32"), 32)
     got = np.fromfile(tmp_path / "bricks.bin", np.uint32)
     pos = 0
     pyr, _ = ds.decode_all_pyramid()
