@@ -39,6 +39,42 @@ struct DecodeLane {
   std::vector<DecodeReq> reqs;  // caller order
 };
 
+// Frames built on the device (direct-mapped pools): the per-brick LOD rule,
+// frustum test and decode requests come from kernels, the op decoder drains
+// a device-built queue, and the whole frame is replayed as a hipGraph.
+struct DevGraph {
+  int form;
+  uint32_t w, h;
+  bool rgba, fixed;
+  const void *cache, *flabel, *frgba, *tfk, *tfr;
+  uint64_t shade_epoch;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+// Device-frame small words (d_small / h_small): queue count, misses of the
+// first and second march, error bits, claim counter; the prefetch's queue
+// count, claim counter and error bits.
+enum { DF_QCOUNT = 0, DF_MISS1 = 1, DF_MISS2 = 2, DF_ERR = 3, DF_CLAIM = 4, DF_AQCOUNT = 8, DF_ACLAIM = 9,
+       DF_AERR = 10, DF_WORDS = 16, DF_HIST = 32 };
+
+struct DevFrames {
+  DecodeReq* d_q = nullptr;     // frame queue (capacity nb)
+  uint32_t* d_qst = nullptr;    // its status words
+  DecodeReq* d_aq = nullptr;    // prefetch queue (capacity nb)
+  uint32_t* d_aqst = nullptr;
+  uint32_t* d_small = nullptr;  // DF_* words, then the prefetch histogram (1028 words)
+  void* d_cam = nullptr;        // MarchCam: camera, look-ahead camera
+  void* d_cam_ahead = nullptr;
+  uint8_t* d_lod = nullptr;     // fixed LODs
+  void* h_cam = nullptr;        // pinned staging of the same
+  void* h_cam_ahead = nullptr;
+  uint8_t* h_lod = nullptr;
+  uint32_t* h_small = nullptr;  // pinned copy of the DF_* words
+  bool aux_pending = false;     // a prefetch decode is in flight on the aux stream
+  std::vector<DevGraph> graphs;
+};
+
 }  // namespace
 
 struct fcs_volume {
@@ -107,6 +143,10 @@ struct fcs_volume {
   unsigned long long* d_csum = nullptr;
   size_t frame_cap = 0;
   size_t frame_px = 0;  // pixels of the last rendered frame
+  // device frames: d_res is changed by the device and is newer than the
+  // host copy (res_lod / res_pyr / res_alloc) until pull_residency
+  DevFrames* df = nullptr;
+  bool res_dev_newer = false;
 
   size_t lbytes() const { return label_bits / 8; }
 };
@@ -373,6 +413,90 @@ int settle_prefetch(fcs_volume* v) {
     if (st[i]) cache_release(v, v->lane_aux.reqs[i].brick, nullptr);
   v->lane_aux.reqs.clear();
   return 0;
+}
+
+// Wait for a device-built prefetch decode on the aux stream (failed requests
+// were made non-resident on the device by the settle kernel).
+int settle_dev_prefetch(fcs_volume* v) {
+  if (!v->df || !v->df->aux_pending) return 0;
+  HIP_TRY(v->ctx, hipEventSynchronize(v->ctx->aux_done));
+  v->df->aux_pending = false;
+  return 0;
+}
+
+// Bring the host copy of the residency (res_lod / res_pyr / res_alloc) up to
+// date after device frames changed the device table. Host residency
+// operations call this first; afterwards host and device agree.
+int pull_residency(fcs_volume* v) {
+  if (int rc = settle_dev_prefetch(v)) return rc;
+  if (!v->res_dev_newer) return 0;
+  fcs_ctx* c = v->ctx;
+  std::vector<uint64_t> w(v->nb);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(w.data(), v->d_res, (size_t)v->nb * 8, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    const uint32_t l = (uint32_t)((w[i] >> FCS_RES_LOD_SHIFT) & 0xFF);
+    if (l == FCS_RES_NONE) {
+      v->res_lod[i] = -1;
+      continue;
+    }
+    v->res_lod[i] = (int16_t)l;
+    v->res_pyr[i] = (w[i] & FCS_RES_PYRAMID) ? 1 : 0;
+    v->res_alloc[i] = Alloc{w[i] & FCS_RES_OFF_MASK,
+                            v->res_pyr[i] ? fcs_pyramid_level_offset(l + 1) : (1ull << (3 * l))};
+  }
+  v->res_dev_newer = false;
+  v->res_published = v->res_epoch;
+  return 0;
+}
+
+// Upload the host residency to the device table when it changed since the
+// last upload. Frames in flight on any stream may read the old table, so the
+// device is synchronised first.
+int push_residency(fcs_volume* v) {
+  fcs_ctx* c = v->ctx;
+  if (!v->d_res) {
+    HIP_TRY(c, hipMalloc((void**)&v->d_res, (size_t)v->nb * 8));
+    v->res_published = 0;
+  }
+  if (v->res_dev_newer || v->res_published == v->res_epoch) return 0;
+  std::vector<uint64_t> words(v->nb);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    uint64_t w = FCS_RES_NONE << FCS_RES_LOD_SHIFT;
+    if (v->res_lod[i] >= 0) {
+      w = (v->res_alloc[i].off & FCS_RES_OFF_MASK) | ((uint64_t)v->res_lod[i] << FCS_RES_LOD_SHIFT);
+      if (v->res_pyr[i]) w |= FCS_RES_PYRAMID;
+    }
+    if (v->empty[i]) w |= FCS_RES_EMPTY;
+    words[i] = w;
+  }
+  HIP_TRY(c, hipDeviceSynchronize());
+  HIP_TRY(c, hipMemcpy(v->d_res, words.data(), words.size() * 8, hipMemcpyHostToDevice));
+  v->res_published = v->res_epoch;
+  return 0;
+}
+
+void drop_dev_graphs(fcs_volume* v) {
+  if (!v->df) return;
+  for (auto& g : v->df->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  v->df->graphs.clear();
+}
+
+void free_dev_frames(fcs_volume* v) {
+  if (!v->df) return;
+  drop_dev_graphs(v);
+  DevFrames* F = v->df;
+  void* dev[] = {F->d_q, F->d_qst, F->d_aq, F->d_aqst, F->d_small, F->d_cam, F->d_cam_ahead, F->d_lod};
+  for (void* q : dev)
+    if (q) (void)hipFree(q);
+  void* host[] = {F->h_cam, F->h_cam_ahead, F->h_lod, F->h_small};
+  for (void* q : host)
+    if (q) (void)hipHostFree(q);
+  delete F;
+  v->df = nullptr;
 }
 
 // Lighting uniform of webgpuSetup.js:205-219: the angle in degrees becomes
@@ -792,25 +916,8 @@ int fcs_volume_publish(fcs_volume* v, fcs_volume_desc* o, bool shading) {
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   if (int rc = settle_prefetch(v)) return rc;
-  if (!v->d_res) {
-    HIP_TRY(c, hipMalloc((void**)&v->d_res, (size_t)v->nb * 8));
-    v->res_published = 0;
-  }
-  if (v->res_published != v->res_epoch) {
-    std::vector<uint64_t> words(v->nb);
-    for (uint32_t i = 0; i < v->nb; ++i) {
-      uint64_t w = FCS_RES_NONE << FCS_RES_LOD_SHIFT;
-      if (v->res_lod[i] >= 0) {
-        w = (v->res_alloc[i].off & FCS_RES_OFF_MASK) | ((uint64_t)v->res_lod[i] << FCS_RES_LOD_SHIFT);
-        if (v->res_pyr[i]) w |= FCS_RES_PYRAMID;
-      }
-      if (v->empty[i]) w |= FCS_RES_EMPTY;
-      words[i] = w;
-    }
-    HIP_TRY(c, hipDeviceSynchronize());
-    HIP_TRY(c, hipMemcpy(v->d_res, words.data(), words.size() * 8, hipMemcpyHostToDevice));
-    v->res_published = v->res_epoch;
-  }
+  if (int rc = pull_residency(v)) return rc;
+  if (int rc = push_residency(v)) return rc;
   o->serial = v->serial;
   o->res = v->d_res;
   o->pal0 = v->d_pal0;
@@ -841,7 +948,9 @@ void fcs_volume_release_device(fcs_volume* v) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)settle_prefetch(v);
+  (void)settle_dev_prefetch(v);
   (void)hipStreamSynchronize(c->stream);
+  free_dev_frames(v);
   void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab, v->d_rlist,
                   v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
                   v->d_misses, v->d_csum, v->d_res, v->d_tf_keys, v->d_tf_rgba, v->d_frgba};
@@ -977,7 +1086,9 @@ int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   if (int rc = settle_prefetch(v)) return rc;
+  if (int rc = settle_dev_prefetch(v)) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_dev_graphs(v);  // they hold the pool's address
   if (v->d_cache) { (void)hipFree(v->d_cache); v->d_cache = nullptr; }
   v->cap = 0;
   v->direct = false;
@@ -991,6 +1102,8 @@ int fcs_cache_reserve(fcs_volume* v, uint64_t voxels) {
 int fcs_cache_clear(fcs_volume* v) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   if (int rc = settle_prefetch(v)) return rc;
+  if (int rc = settle_dev_prefetch(v)) return rc;
+  v->res_dev_newer = false;  // every brick dropped: the host copy is current again
   v->bump = 0;
   for (auto& f : v->free_lists) f.clear();
   std::fill(v->res_lod.begin(), v->res_lod.end(), (int16_t)-1);
@@ -999,8 +1112,9 @@ int fcs_cache_clear(fcs_volume* v) {
   return 0;
 }
 
-int64_t fcs_cache_resident(const fcs_volume* v, uint32_t* bricks, uint8_t* lods, uint64_t cap) {
+int64_t fcs_cache_resident(fcs_volume* v, uint32_t* bricks, uint8_t* lods, uint64_t cap) {
   if (!v || !v->ctx) return FCS_E_INVALID;
+  if (int rc = pull_residency(v)) return rc;
   uint64_t n = 0;
   for (uint32_t i = 0; i < v->nb; ++i) {
     if (v->res_lod[i] < 0) continue;
@@ -1030,6 +1144,7 @@ static int request_bricks_impl(fcs_volume* v, const uint32_t* brick, const uint8
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   if (int rc = settle_prefetch(v)) return rc;
+  if (int rc = pull_residency(v)) return rc;
   if (no_alloc) {
     if (!v->rans_ready) return fail(c, FCS_E_INVALID, "fcs_decode_rans has not run");
     if (!v->d_cache) return fail(c, FCS_E_INVALID, "no brick cache pool reserved");
@@ -1449,26 +1564,16 @@ void shade_args(const fcs_volume* v, bool rgba, MarchArgs<LT>& a) {
   a.diffuse = v->diffuse;
 }
 
-// Upload the frame's per-brick metadata (LOD, empty, resident at that LOD,
-// cache offset) and march every pixel on the context stream (not waited for).
-int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h,
-               bool rgba) {
+// Frame output buffers of at least npx pixels (RGBA when asked), the visited
+// mask and the miss counter. Device-frame graphs hold their addresses: they
+// are dropped when a buffer moves.
+int ensure_frame_buffers(fcs_volume* v, size_t npx, bool rgba) {
   fcs_ctx* c = v->ctx;
-  if (rgba)
-    if (int rc = ensure_tf(v)) return rc;
-  std::vector<uint32_t> meta(v->nb);
-  std::vector<uint64_t> offs(v->nb, 0);
-  for (uint32_t i = 0; i < v->nb; ++i) {
-    uint32_t m = lods[i];
-    if (v->empty[i]) m |= FCS_META_EMPTY;
-    if (resident_at(v, i, lods[i])) {
-      m |= FCS_META_RESIDENT;
-      offs[i] = v->res_alloc[i].off + (v->res_pyr[i] ? fcs_pyramid_level_offset(lods[i]) : 0);
-    }
-    meta[i] = m;
-  }
-  const size_t npx = (size_t)w * h;
   if (npx > v->frame_cap) {
+    if (v->df && !v->df->graphs.empty()) {
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      drop_dev_graphs(v);
+    }
     void* ptrs[] = {v->d_flabel, v->d_fvoxel, v->d_ft, v->d_frgba};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
@@ -1487,6 +1592,28 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
     HIP_TRY(c, hipMalloc((void**)&v->d_visited, (size_t)v->nb * 4));
     HIP_TRY(c, hipMalloc((void**)&v->d_misses, 64));
   }
+  return 0;
+}
+
+// Upload the frame's per-brick metadata (LOD, empty, resident at that LOD,
+// cache offset) and march every pixel on the context stream (not waited for).
+int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>& lods, uint32_t w, uint32_t h,
+               bool rgba) {
+  fcs_ctx* c = v->ctx;
+  if (rgba)
+    if (int rc = ensure_tf(v)) return rc;
+  std::vector<uint32_t> meta(v->nb);
+  std::vector<uint64_t> offs(v->nb, 0);
+  for (uint32_t i = 0; i < v->nb; ++i) {
+    uint32_t m = lods[i];
+    if (v->empty[i]) m |= FCS_META_EMPTY;
+    if (resident_at(v, i, lods[i])) {
+      m |= FCS_META_RESIDENT;
+      offs[i] = v->res_alloc[i].off + (v->res_pyr[i] ? fcs_pyramid_level_offset(lods[i]) : 0);
+    }
+    meta[i] = m;
+  }
+  if (int rc = ensure_frame_buffers(v, (size_t)w * h, rgba)) return rc;
   HIP_TRY(c, hipMemcpyAsync(v->d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipMemcpyAsync(v->d_offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, fcs_launch_zero(v->d_visited, v->nb, c->stream));
@@ -1533,6 +1660,264 @@ int finish_frame(fcs_volume* v, const std::vector<uint8_t>& lods, uint32_t w, ui
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Frames built on the device (direct-mapped pools). Two forms:
+//   plain    (fcs_render): frame setup (LOD rule, frustum test) queues every
+//            non-empty brick in the frustum that is not resident at its LOD;
+//            the op decoder drains the queue; the march follows.
+//   visible  (fcs_render_visible): setup, march with what is resident, queue
+//            the bricks rays entered but found missing, decode, and march
+//            again only if there were misses (a second pass's rays enter a
+//            subset of the bricks the first pass's rays entered, so two passes
+//            see every brick they need).
+// The sequence is captured once per (form, frame size, outputs) as a hipGraph;
+// the camera (and fixed LODs) reach it through pinned staging buffers.
+enum { FORM_PLAIN = 0, FORM_VISIBLE = 1 };
+
+int ensure_dev_frames(fcs_volume* v) {
+  if (v->df) return 0;
+  fcs_ctx* c = v->ctx;
+  DevFrames* F = new DevFrames();
+  v->df = F;
+  const size_t nb = v->nb;
+  HIP_TRY(c, hipMalloc((void**)&F->d_q, nb * sizeof(DecodeReq)));
+  HIP_TRY(c, hipMalloc((void**)&F->d_qst, nb * 4));
+  HIP_TRY(c, hipMalloc((void**)&F->d_aq, nb * sizeof(DecodeReq)));
+  HIP_TRY(c, hipMalloc((void**)&F->d_aqst, nb * 4));
+  HIP_TRY(c, hipMalloc((void**)&F->d_small, (DF_HIST + 1028) * 4));
+  HIP_TRY(c, hipMalloc(&F->d_cam, sizeof(MarchCam)));
+  HIP_TRY(c, hipMalloc(&F->d_cam_ahead, sizeof(MarchCam)));
+  HIP_TRY(c, hipMalloc((void**)&F->d_lod, nb));
+  HIP_TRY(c, hipHostMalloc(&F->h_cam, sizeof(MarchCam), hipHostMallocDefault));
+  HIP_TRY(c, hipHostMalloc(&F->h_cam_ahead, sizeof(MarchCam), hipHostMallocDefault));
+  HIP_TRY(c, hipHostMalloc((void**)&F->h_lod, nb, hipHostMallocDefault));
+  HIP_TRY(c, hipHostMalloc((void**)&F->h_small, DF_WORDS * 4, hipHostMallocDefault));
+  return 0;
+}
+
+// The op decoder over a device-built queue (count in device memory).
+int issue_queue_decode(fcs_volume* v, DecodeReq* q, uint32_t* status, uint32_t* qcount, uint32_t* claim,
+                       int split, hipStream_t s) {
+  fcs_ctx* c = v->ctx;
+  const int ksplit = split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG;
+  if (v->label_bits == 32) {
+    OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, q, (uint32_t*)v->d_cache, status, claim,
+                        v->nb, v->nb, v->L, 0u, qcount};
+    HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, s, nullptr, ksplit, FCS_OPS_HEAD_OFF));
+  } else {
+    OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, q, (uint64_t*)v->d_cache, status, claim,
+                        v->nb, v->nb, v->L, 0u, qcount};
+    HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, s, nullptr, ksplit, FCS_OPS_HEAD_OFF));
+  }
+  QueueSettleArgs sa{q, qcount, status, v->d_res, v->d_meta, v->df->d_small + DF_ERR, v->nb};
+  HIP_TRY(c, fcs_launch_queue_settle(sa, s));
+  return 0;
+}
+
+template <typename LT>
+int dev_march(fcs_volume* v, uint32_t w, uint32_t h, bool rgba, uint32_t* misses, const uint32_t* run_if) {
+  fcs_ctx* c = v->ctx;
+  MarchCam zc;
+  memset(&zc, 0, sizeof zc);
+  const float aspect = (float)((double)w / (double)h);
+  MarchArgs<LT> a{zc, (const MarchCam*)v->df->d_cam, v->d_meta, v->d_offs, (const LT*)v->d_pal0,
+                  (const LT*)v->d_cache, (LT*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, misses,
+                  w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
+  shade_args(v, rgba, a);
+  a.run_if = run_if;
+  HIP_TRY(c, fcs_launch_march<LT>(a, c->stream));
+  return 0;
+}
+
+// The device work of one frame, in stream order (captured into a graph).
+int enqueue_dev_frame(fcs_volume* v, int form, uint32_t w, uint32_t h, bool rgba, bool fixed, int split) {
+  fcs_ctx* c = v->ctx;
+  DevFrames& F = *v->df;
+  hipStream_t s = c->stream;
+  uint32_t* sm = F.d_small;
+  HIP_TRY(c, hipMemcpyAsync(F.d_cam, F.h_cam, sizeof(MarchCam), hipMemcpyHostToDevice, s));
+  if (fixed) HIP_TRY(c, hipMemcpyAsync(F.d_lod, F.h_lod, v->nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, fcs_launch_zero(sm, 8, s));
+  const bool plain = form == FORM_PLAIN;
+  FrameSetupArgs fa{v->d_res, (const MarchCam*)F.d_cam, fixed ? F.d_lod : nullptr, v->d_meta, v->d_offs,
+                    v->nb, v->b, v->L, w, h, plain ? v->d_res : nullptr, plain ? F.d_q : nullptr,
+                    plain ? sm + DF_QCOUNT : nullptr, v->stride};
+  HIP_TRY(c, fcs_launch_frame_setup(fa, s, plain));
+  if (plain)
+    if (int rc = issue_queue_decode(v, F.d_q, F.d_qst, sm + DF_QCOUNT, sm + DF_CLAIM, split, s)) return rc;
+  HIP_TRY(c, fcs_launch_zero(v->d_visited, v->nb, s));
+  auto march = [&](uint32_t* misses, const uint32_t* run_if) {
+    return v->label_bits == 32 ? dev_march<uint32_t>(v, w, h, rgba, misses, run_if)
+                               : dev_march<uint64_t>(v, w, h, rgba, misses, run_if);
+  };
+  if (int rc = march(sm + DF_MISS1, nullptr)) return rc;
+  if (!plain) {
+    MissEnqueueArgs ma{sm + DF_MISS1, v->d_visited, v->d_meta, v->d_offs, v->d_res, F.d_q, sm + DF_QCOUNT,
+                       v->nb, v->stride};
+    HIP_TRY(c, fcs_launch_miss_enqueue(ma, s));
+    if (int rc = issue_queue_decode(v, F.d_q, F.d_qst, sm + DF_QCOUNT, sm + DF_CLAIM, split, s)) return rc;
+    HIP_TRY(c, fcs_launch_zero_if(v->d_visited, v->nb, sm + DF_MISS1, s));
+    if (int rc = march(sm + DF_MISS2, sm + DF_MISS1)) return rc;
+  }
+  HIP_TRY(c, hipMemcpyAsync(F.h_small, sm, 8 * 4, hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+int dev_frame_graph(fcs_volume* v, int form, uint32_t w, uint32_t h, bool rgba, bool fixed, DevGraph** out) {
+  fcs_ctx* c = v->ctx;
+  DevFrames& F = *v->df;
+  const int split = ops_split_of(v);
+  const void* tfk = rgba ? (const void*)v->d_tf_keys : nullptr;
+  const void* tfr = rgba ? (const void*)v->d_tf_rgba : nullptr;
+  const uint64_t se = rgba ? v->shade_epoch : 0;
+  for (auto& g : F.graphs)
+    if (g.form == form && g.w == w && g.h == h && g.rgba == rgba && g.fixed == fixed && g.cache == v->d_cache &&
+        g.flabel == v->d_flabel && g.frgba == (rgba ? v->d_frgba : nullptr) && g.tfk == tfk && g.tfr == tfr &&
+        g.shade_epoch == se) {
+      *out = &g;
+      return 0;
+    }
+  // graphs of this shape with stale addresses or shading can never be replayed
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (auto& g : F.graphs)
+    if (g.form == form && g.w == w && g.h == h && g.rgba == rgba && g.fixed == fixed) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+    }
+  F.graphs.erase(std::remove_if(F.graphs.begin(), F.graphs.end(), [](const DevGraph& g) { return !g.exec; }),
+                 F.graphs.end());
+  // the op decoder queries its occupancy on its first launch: not inside a capture
+  HIP_TRY(c, v->label_bits == 32 ? fcs_ops_prepare<uint32_t>(v->L, split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG)
+                                 : fcs_ops_prepare<uint64_t>(v->L, split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG));
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_dev_frame(v, form, w, h, rgba, fixed, split);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(c, e);
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    HIP_TRY(c, ei);
+  }
+  F.graphs.push_back(DevGraph{form, w, h, rgba, fixed, v->d_cache, v->d_flabel, rgba ? v->d_frgba : nullptr, tfk, tfr,
+                              se, g, x});
+  *out = &F.graphs.back();
+  return 0;
+}
+
+// One device-built frame; outputs copied back as fcs_render's.
+int dev_frame(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, uint32_t w, uint32_t h, int form,
+              fcs_frame* out) {
+  fcs_ctx* c = v->ctx;
+  if (int rc = settle_prefetch(v)) return rc;
+  if (!v->rans_ready) {
+    const int rc = fcs_decode_rans(v, nullptr);
+    if (rc && rc != FCS_E_CORRUPT) return rc;
+  }
+  if (int rc = ensure_dev_frames(v)) return rc;
+  DevFrames& F = *v->df;
+  const bool rgba = out->rgba != nullptr;
+  if (rgba)
+    if (int rc = ensure_tf(v)) return rc;
+  const size_t npx = (size_t)w * h;
+  if (int rc = ensure_frame_buffers(v, npx, rgba)) return rc;
+  if (int rc = push_residency(v)) return rc;
+  memcpy(F.h_cam, cam, sizeof(MarchCam));
+  if (brick_lod) memcpy(F.h_lod, brick_lod, v->nb);
+  // a prefetch decode issued after the previous frame finishes first
+  if (F.aux_pending) {
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
+    F.aux_pending = false;
+  }
+  DevGraph* g = nullptr;
+  if (int rc = dev_frame_graph(v, form, w, h, rgba, brick_lod != nullptr, &g)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  HIP_TRY(c, hipGraphLaunch(g->exec, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  v->res_dev_newer = true;
+  if (out->label) HIP_TRY(c, hipMemcpyAsync(out->label, v->d_flabel, npx * v->lbytes(), hipMemcpyDeviceToHost, c->stream));
+  if (out->voxel) HIP_TRY(c, hipMemcpyAsync(out->voxel, v->d_fvoxel, npx * 8, hipMemcpyDeviceToHost, c->stream));
+  if (out->t_hit) HIP_TRY(c, hipMemcpyAsync(out->t_hit, v->d_ft, npx * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out->rgba) HIP_TRY(c, hipMemcpyAsync(out->rgba, v->d_frgba, npx * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out->brick_visited)
+    HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
+  std::vector<uint32_t> meta;
+  if (out->brick_lod_used) {
+    meta.resize(v->nb);
+    HIP_TRY(c, hipMemcpyAsync(meta.data(), v->d_meta, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < meta.size(); ++i) out->brick_lod_used[i] = (uint8_t)(meta[i] & FCS_META_LOD_MASK);
+  v->frame_px = npx;
+  float ms = 0;
+  HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  out->bricks_decoded = F.h_small[DF_QCOUNT];
+  out->decode_ms = 0.0f;
+  out->march_ms = ms;
+  c->timing.march_ms = ms;
+  if (F.h_small[DF_ERR])
+    return fail(c, FCS_E_CORRUPT, "a brick decoded for the frame failed (status bits in the request statuses)");
+  const uint32_t left = form == FORM_PLAIN ? F.h_small[DF_MISS1] : (F.h_small[DF_MISS1] ? F.h_small[DF_MISS2] : 0u);
+  if (left) return fail(c, FCS_E_CORRUPT, "rays entered bricks that could not be made resident");
+  return 0;
+}
+
+// Prefetch decodes chosen on the device from the network's probabilities,
+// issued on the aux stream after everything queued on the context stream;
+// the next frame waits for them.
+int dev_prefetch(fcs_volume* v, const float* d_probs, const fcs_camera* cam_ahead, uint32_t w, uint32_t h,
+                 float threshold, uint32_t max_prefetch, uint32_t* out_prefetched, uint32_t* out_bricks) {
+  fcs_ctx* c = v->ctx;
+  DevFrames& F = *v->df;
+  uint32_t* sm = F.d_small;
+  const uint32_t cap = std::min<uint32_t>(max_prefetch, v->nb);
+  memcpy(F.h_cam_ahead, cam_ahead, sizeof(MarchCam));
+  HIP_TRY(c, hipEventRecord(c->main_mark, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->aux, c->main_mark, 0));
+  HIP_TRY(c, hipMemcpyAsync(F.d_cam_ahead, F.h_cam_ahead, sizeof(MarchCam), hipMemcpyHostToDevice, c->aux));
+  HIP_TRY(c, fcs_launch_zero(sm + DF_AQCOUNT, 8, c->aux));
+  PrefetchArgs pa{d_probs, (const MarchCam*)F.d_cam_ahead, v->d_res, F.d_aq, sm + DF_AQCOUNT, sm + DF_HIST,
+                  threshold, cap, v->nb, v->b, v->L, w, h, v->stride};
+  HIP_TRY(c, fcs_launch_prefetch_enqueue(pa, c->aux));
+  uint32_t n = 0;
+  HIP_TRY(c, hipMemcpyAsync(&n, sm + DF_AQCOUNT, 4, hipMemcpyDeviceToHost, c->aux));
+  std::vector<DecodeReq> q;
+  if (out_bricks && cap) {
+    q.resize(cap);
+    HIP_TRY(c, hipMemcpyAsync(q.data(), F.d_aq, cap * sizeof(DecodeReq), hipMemcpyDeviceToHost, c->aux));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->aux));
+  n = std::min(n, cap);
+  if (n) {
+    const int split = ops_split_of(v);
+    const int ksplit = split == FCS_OPS_SPLIT_4WG ? FCS_OPS_4WG : FCS_OPS_3WG;
+    if (v->label_bits == 32) {
+      OpsArgs<uint32_t> a{v->d_bricks, v->d_nib, (const uint32_t*)v->d_pal, F.d_aq, (uint32_t*)v->d_cache, F.d_aqst,
+                          sm + DF_ACLAIM, n, v->nb, v->L, 0u, nullptr};
+      HIP_TRY(c, fcs_launch_ops<uint32_t>(a, (uint32_t)c->num_cu, c->aux, nullptr, ksplit, FCS_OPS_HEAD_AUTO));
+    } else {
+      OpsArgs<uint64_t> a{v->d_bricks, v->d_nib, (const uint64_t*)v->d_pal, F.d_aq, (uint64_t*)v->d_cache, F.d_aqst,
+                          sm + DF_ACLAIM, n, v->nb, v->L, 0u, nullptr};
+      HIP_TRY(c, fcs_launch_ops<uint64_t>(a, (uint32_t)c->num_cu, c->aux, nullptr, ksplit, FCS_OPS_HEAD_AUTO));
+    }
+    QueueSettleArgs sa{F.d_aq, sm + DF_AQCOUNT, F.d_aqst, v->d_res, nullptr, sm + DF_AERR, n};
+    HIP_TRY(c, fcs_launch_queue_settle(sa, c->aux));
+    HIP_TRY(c, hipEventRecord(c->aux_done, c->aux));
+    F.aux_pending = true;
+    v->res_dev_newer = true;
+  }
+  if (out_prefetched) *out_prefetched = n;
+  if (out_bricks)
+    for (uint32_t i = 0; i < cap; ++i) out_bricks[i] = i < n ? q[i].brick : ~0u;
+  return 0;
+}
+
 int decode_list(fcs_volume* v, const std::vector<uint32_t>& qb, const std::vector<uint8_t>& ql, fcs_frame* out) {
   if (qb.empty()) return 0;
   int rc = fcs_request_bricks(v, qb.data(), ql.data(), (uint32_t)qb.size(), nullptr, nullptr);
@@ -1551,6 +1936,7 @@ static int render_body(fcs_volume* v, const fcs_camera* cam, const uint8_t* bric
   if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (v->direct) return dev_frame(v, cam, brick_lod, w, h, FORM_PLAIN, out);
   std::vector<uint8_t> lods;
   if (int rc = frame_lods(v, cam, brick_lod, w, h, lods)) return rc;
   // decodes issued ahead on the aux stream finish before this frame's work
@@ -1583,6 +1969,7 @@ static int render_visible_body(fcs_volume* v, const fcs_camera* cam, const uint8
   if (!v || !v->ctx || !cam || !out || w == 0 || h == 0) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (v->direct) return dev_frame(v, cam, brick_lod, w, h, FORM_VISIBLE, out);
   std::vector<uint8_t> lods;
   if (int rc = frame_lods(v, cam, brick_lod, w, h, lods)) return rc;
   if (c->aux_pending) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
@@ -1632,6 +2019,7 @@ static int prefetch_bricks_body(fcs_volume* v, const uint32_t* brick, const uint
   }
   if (int rc = ensure_cache(v)) return rc;
   if (int rc = settle_prefetch(v)) return rc;  // one prefetch batch in flight
+  if (int rc = pull_residency(v)) return rc;
   std::vector<DecodeReq> reqs;
   std::vector<Alloc> deferred;
   std::vector<uint8_t> seen(v->nb, 0);
@@ -1714,6 +2102,9 @@ static int render_prefetch_body(fcs_volume* v, fcs_mlp* m, const fcs_camera* cam
   fcs_camera_features(v, cam, feat);
   const float* d_probs = nullptr;
   if (int rc = fcs_mlp_predict_dev(m, feat, 12, v->d_visited, &d_probs)) return rc;
+  if (v->direct)
+    return dev_prefetch(v, d_probs, cam_ahead ? cam_ahead : cam, w, h, threshold, max_prefetch, out_prefetched,
+                        out_bricks);
   std::vector<float> probs(v->nb);
   HIP_TRY(c, hipMemcpyAsync(probs.data(), d_probs, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));

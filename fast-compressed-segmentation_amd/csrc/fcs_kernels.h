@@ -36,10 +36,11 @@ struct OpsArgs {
   LT* out;
   uint32_t* status;
   uint32_t* counter;  // work counter (reset by the launcher)
-  uint32_t nreq;
+  uint32_t nreq;      // requests (with nreq_dev: the capacity of the request list)
   uint32_t nbricks;
   uint32_t L;
   uint32_t head;      // set by the launcher: a head pass left levels 1..3 in the outputs
+  const uint32_t* nreq_dev;  // nullable: the request count, read from device memory (a device-built queue)
 };
 
 // per-brick frame metadata for the marcher
@@ -81,6 +82,7 @@ struct MarchArgs {
   uint32_t tf_mask;         // slots - 1 (a power of two minus one)
   float light[3];           // light direction (towards the light, not normalised)
   float ambient, diffuse;
+  const uint32_t* run_if;   // nullable: the launch does nothing when *run_if == 0
 };
 
 // Transfer function hash slot of a label (SplitMix64 finaliser).
@@ -110,9 +112,67 @@ struct FrameSetupArgs {
   uint32_t* meta;
   uint64_t* offs;
   uint32_t nb, b, L, w, h;
+  // On-demand decode (a device-owned direct-mapped cache; all nullable):
+  // every non-empty brick inside the view frustum that is not resident at its
+  // LOD gets a request (brick, LOD, slot brick * stride) in `queue`, its
+  // residency word in `res_w` is set to that LOD and it is marked resident
+  // for the march that follows the decode.
+  uint64_t* res_w;
+  DecodeReq* queue;
+  uint32_t* qcount;
+  uint64_t stride;
 };
 
-hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s);
+// Requests for the bricks a march entered but found not resident at their
+// LOD (visited && !resident, non-empty, LOD > 0), when *misses != 0: queued,
+// marked resident in meta/offs and in the residency table (direct slots).
+struct MissEnqueueArgs {
+  const uint32_t* misses;
+  const uint32_t* visited;
+  uint32_t* meta;
+  uint64_t* offs;
+  uint64_t* res_w;
+  DecodeReq* queue;
+  uint32_t* qcount;
+  uint32_t nb;
+  uint64_t stride;
+};
+
+// After a device-queued decode: requests that failed are OR-ed into *err and
+// their bricks made non-resident again (residency table and meta).
+struct QueueSettleArgs {
+  const DecodeReq* queue;
+  const uint32_t* qcount;
+  const uint32_t* status;
+  uint64_t* res_w;
+  uint32_t* meta;     // nullable
+  uint32_t* err;
+  uint32_t cap;
+};
+
+// Prefetch candidates from the network's probabilities: non-empty bricks
+// with p >= threshold whose LOD by the rule at the look-ahead camera is > 0
+// and that are not resident at it; at most `cap` of them, the most probable
+// first (probabilities bucketed to 1/1024), queued for a decode into their
+// direct slots and marked resident.
+struct PrefetchArgs {
+  const float* probs;
+  const MarchCam* cam;  // look-ahead camera (device)
+  uint64_t* res_w;
+  DecodeReq* queue;
+  uint32_t* qcount;
+  uint32_t* hist;       // 1024 + 4 counters, cleared by the launcher
+  float threshold;
+  uint32_t cap, nb, b, L, w, h;
+  uint64_t stride;
+};
+
+hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s, bool frustum = false);
+hipError_t fcs_launch_miss_enqueue(const MissEnqueueArgs& a, hipStream_t s);
+hipError_t fcs_launch_queue_settle(const QueueSettleArgs& a, hipStream_t s);
+hipError_t fcs_launch_prefetch_enqueue(const PrefetchArgs& a, hipStream_t s);
+// zero p[0..n) when *flag != 0
+hipError_t fcs_launch_zero_if(uint32_t* p, uint64_t n, const uint32_t* flag, hipStream_t s);
 hipError_t fcs_launch_rans(const RansArgs& a, int mode, hipStream_t s);
 size_t fcs_ops_lds_bytes(uint32_t L);
 // Op decoder LDS splits for bricks up to 32^3 (64^3 bricks have one).
@@ -125,6 +185,10 @@ size_t fcs_ops_lds_bytes(uint32_t L);
 template <typename LT>
 hipError_t fcs_launch_ops(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out = nullptr,
                           int split = FCS_OPS_3WG, int head = 0);
+// The launch configuration's one-time set-up (LDS attribute, occupancy
+// query), done before a stream capture that launches the op decoder.
+template <typename LT>
+hipError_t fcs_ops_prepare(uint32_t L, int split);
 // First-hit march; with a.out_rgba set, the composited shaded march that also
 // writes the first-hit outputs.
 template <typename LT>

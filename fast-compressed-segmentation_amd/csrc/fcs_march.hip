@@ -175,6 +175,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= a.w || py >= a.h) return;
+  if (a.run_if && *a.run_if == 0u) return;
   if (a.cam_dev) a.cam = *a.cam_dev;
   const size_t pix = (size_t)py * a.w + px;
   LT hit_label = 0;
@@ -436,48 +437,242 @@ namespace {
 // largest t with (focal * b/2^t)/d >= 1. Division and square root are the
 // correctly rounded IEEE operations on both sides and nothing is contracted,
 // so device and host pick the same LOD for every brick.
+__device__ __forceinline__ uint32_t lod_rule(const MarchCam& c, uint32_t i, uint32_t b, uint32_t L, uint32_t w,
+                                             uint32_t h) {
+  const double sw = (double)(w < h ? w : h);
+  const double focal = (sw / 2.0) / (double)c.tan_half_fov;
+  const double half = (double)b / 2.0;
+  const double cx = (double)fcs_unmorton_x(i) * (double)b + half;
+  const double cy = (double)fcs_unmorton_y(i) * (double)b + half;
+  const double cz = (double)fcs_unmorton_z(i) * (double)b + half;
+  const double dx = cx - (double)c.position[0], dy = cy - (double)c.position[1], dz = cz - (double)c.position[2];
+  double d = sqrt(dx * dx + dy * dy + dz * dz);
+  if (d < 0.1) d = 0.1;
+  uint32_t lod = 0;
+  for (uint32_t l = 0; l <= L; ++l) {
+    const double voxel = (double)b / (double)(1u << l);  // exact power of two
+    if ((focal * voxel) / d >= 1.0) lod = l; else break;
+  }
+  return lod;
+}
+
+// View-frustum test of brick i (culling.js:87-171 with exact perspective
+// planes): the brick's box grown by one voxel lies on the inner side of the
+// four side planes and in front of the camera. Orthographic cameras keep
+// every brick. float64, the same operations as the host test.
+__device__ __forceinline__ bool in_frustum(const MarchCam& c, uint32_t i, uint32_t b, uint32_t w, uint32_t h) {
+  if (c.ortho_half_height > 0.0f || !(c.tan_half_fov > 0.0f)) return true;
+  const double A = (double)w / (double)h, T = c.tan_half_fov;
+  double n[5][3];
+  for (int k = 0; k < 3; ++k) {
+    const double f = c.forward[k], r = c.right[k], u = c.up[k];
+    n[0][k] = r - A * T * f;
+    n[1][k] = -r - A * T * f;
+    n[2][k] = u - T * f;
+    n[3][k] = -u - T * f;
+    n[4][k] = -f;
+  }
+  const double bd = b;
+  const uint32_t bc[3] = {fcs_unmorton_x(i), fcs_unmorton_y(i), fcs_unmorton_z(i)};
+  for (int q = 0; q < 5; ++q) {
+    double d = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double lo = bc[k] * bd - 1.0, hi = (bc[k] + 1) * bd + 1.0;
+      d += n[q][k] * ((n[q][k] >= 0 ? lo : hi) - (double)c.position[k]);
+    }
+    if (d > 0) return false;
+  }
+  return true;
+}
+
+// Wave-aggregated slot in a device queue: one atomic per wave.
+__device__ __forceinline__ uint32_t queue_slot(bool want, uint32_t* count) {
+  const uint64_t m = __ballot(want);
+  if (!m) return 0xFFFFFFFFu;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  const uint64_t below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+  return want ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint64_t res_word(uint64_t off, uint32_t lod, uint64_t old) {
+  return (off & FCS_RES_OFF_MASK) | ((uint64_t)lod << FCS_RES_LOD_SHIFT) | (old & FCS_RES_EMPTY);
+}
+
+template <bool kFrustum>
 __global__ __launch_bounds__(256) void frame_setup_kernel(FrameSetupArgs a) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.nb) return;
-  uint32_t lod;
-  if (a.lod_fixed) {
-    lod = a.lod_fixed[i] < a.L ? a.lod_fixed[i] : a.L;
-  } else {
-    const MarchCam& c = *a.cam;
-    const double sw = (double)(a.w < a.h ? a.w : a.h);
-    const double focal = (sw / 2.0) / (double)c.tan_half_fov;
-    const double half = (double)a.b / 2.0;
-    const double cx = (double)fcs_unmorton_x(i) * (double)a.b + half;
-    const double cy = (double)fcs_unmorton_y(i) * (double)a.b + half;
-    const double cz = (double)fcs_unmorton_z(i) * (double)a.b + half;
-    const double dx = cx - (double)c.position[0], dy = cy - (double)c.position[1], dz = cz - (double)c.position[2];
-    double d = sqrt(dx * dx + dy * dy + dz * dz);
-    if (d < 0.1) d = 0.1;
-    lod = 0;
-    for (uint32_t l = 0; l <= a.L; ++l) {
-      const double voxel = (double)a.b / (double)(1u << l);  // exact power of two
-      if ((focal * voxel) / d >= 1.0) lod = l; else break;
-    }
+  const bool live = i < a.nb;
+  uint32_t lod = 0;
+  uint64_t r = 0;
+  bool resident = false, pyr = false, need = false;
+  if (live) {
+    if (a.lod_fixed) lod = a.lod_fixed[i] < a.L ? a.lod_fixed[i] : a.L;
+    else lod = lod_rule(*a.cam, i, a.b, a.L, a.w, a.h);
+    r = a.res[i];
+    const uint32_t rl = (uint32_t)((r >> FCS_RES_LOD_SHIFT) & 0xFF);
+    pyr = (r & FCS_RES_PYRAMID) != 0;
+    resident = rl != FCS_RES_NONE && (pyr ? rl >= lod : rl == lod);
+    need = a.queue && !resident && lod > 0 && !(r & FCS_RES_EMPTY) &&
+           (!kFrustum || in_frustum(*a.cam, i, a.b, a.w, a.h));
   }
-  const uint64_t r = a.res[i];
-  const uint32_t rl = (uint32_t)((r >> FCS_RES_LOD_SHIFT) & 0xFF);
-  const bool pyr = (r & FCS_RES_PYRAMID) != 0;
-  const bool resident = rl != FCS_RES_NONE && (pyr ? rl >= lod : rl == lod);
+  // every lane of the wave takes part in the ballot
+  const uint32_t slot = a.queue ? queue_slot(need, a.qcount) : 0xFFFFFFFFu;
+  if (!live) return;
   uint32_t m = lod;
   if (r & FCS_RES_EMPTY) m |= FCS_META_EMPTY;
   uint64_t off = 0;
-  if (resident) {
+  if (need) {
+    off = (uint64_t)i * a.stride;
+    a.queue[slot] = DecodeReq{i, lod, off};
+    a.res_w[i] = res_word(off, lod, r);
+    m |= FCS_META_RESIDENT;
+  } else if (resident) {
     m |= FCS_META_RESIDENT;
     off = (r & FCS_RES_OFF_MASK) + (pyr ? fcs_pyramid_level_offset(lod) : 0);
   }
   a.meta[i] = m;
   a.offs[i] = off;
 }
+
+__global__ __launch_bounds__(256) void miss_enqueue_kernel(MissEnqueueArgs a) {
+  if (*a.misses == 0u) return;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool need = false;
+  uint32_t m = 0;
+  if (i < a.nb) {
+    m = a.meta[i];
+    need = a.visited[i] && !(m & (FCS_META_RESIDENT | FCS_META_EMPTY)) && (m & FCS_META_LOD_MASK) > 0;
+  }
+  const uint32_t slot = queue_slot(need, a.qcount);
+  if (!need) return;
+  const uint32_t lod = m & FCS_META_LOD_MASK;
+  const uint64_t off = (uint64_t)i * a.stride;
+  a.queue[slot] = DecodeReq{i, lod, off};
+  a.res_w[i] = res_word(off, lod, a.res_w[i]);
+  a.meta[i] = m | FCS_META_RESIDENT;
+  a.offs[i] = off;
+}
+
+__global__ __launch_bounds__(256) void queue_settle_kernel(QueueSettleArgs a) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t n = min(*a.qcount, a.cap);
+  if (k >= n) return;
+  const uint32_t st = a.status[k];
+  if (!st) return;
+  atomicOr(a.err, st);
+  const uint32_t i = a.queue[k].brick;
+  a.res_w[i] = (FCS_RES_NONE << FCS_RES_LOD_SHIFT) | (a.res_w[i] & FCS_RES_EMPTY);
+  if (a.meta) a.meta[i] &= ~FCS_META_RESIDENT;
+}
+
+// Prefetch candidate of brick i: its LOD at the look-ahead camera and the
+// probability bucket (1024 buckets), or false.
+__device__ __forceinline__ bool prefetch_candidate(const PrefetchArgs& a, uint32_t i, uint32_t* lod,
+                                                   uint32_t* bucket) {
+  const float p = a.probs[i];
+  if (!(p >= a.threshold)) return false;
+  const uint64_t r = a.res_w[i];
+  if (r & FCS_RES_EMPTY) return false;
+  const uint32_t t = lod_rule(*a.cam, i, a.b, a.L, a.w, a.h);
+  if (t == 0) return false;
+  const uint32_t rl = (uint32_t)((r >> FCS_RES_LOD_SHIFT) & 0xFF);
+  const bool pyr = (r & FCS_RES_PYRAMID) != 0;
+  if (rl != FCS_RES_NONE && (pyr ? rl >= t : rl == t)) return false;
+  *lod = t;
+  *bucket = min((uint32_t)(p * 1024.0f), 1023u);
+  return true;
+}
+
+__global__ __launch_bounds__(256) void prefetch_hist_kernel(PrefetchArgs a) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t t, bk;
+  if (i < a.nb && prefetch_candidate(a, i, &t, &bk)) atomicAdd(&a.hist[bk], 1u);
+}
+
+// One wave: the lowest bucket still taken whole (hist[1024]) and how many of
+// the bucket below it fit (hist[1025]), from the most probable bucket down.
+__global__ void prefetch_cut_kernel(PrefetchArgs a) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0, cut = 0, part = 0;
+  for (int k = 1023; k >= 0; --k) {
+    const uint32_t c = a.hist[k];
+    if (acc + c > a.cap) {
+      cut = (uint32_t)k + 1u;
+      part = a.cap - acc;
+      break;
+    }
+    acc += c;
+  }
+  a.hist[1024] = cut;
+  a.hist[1025] = part;
+}
+
+__global__ __launch_bounds__(256) void prefetch_enqueue_kernel(PrefetchArgs a) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t t = 0, bk = 0;
+  bool want = i < a.nb && prefetch_candidate(a, i, &t, &bk);
+  const uint32_t cut = a.hist[1024];
+  if (want && bk < cut) {
+    // the partly taken bucket: the first `part` claimants
+    want = bk + 1u == cut && atomicAdd(&a.hist[1026], 1u) < a.hist[1025];
+  }
+  const uint32_t slot = queue_slot(want, a.qcount);
+  if (!want) return;
+  const uint64_t off = (uint64_t)i * a.stride;
+  a.queue[slot] = DecodeReq{i, t, off};
+  a.res_w[i] = res_word(off, t, a.res_w[i]);
+}
+
+__global__ void zero_u32_kernel_m(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void zero_if_kernel(uint32_t* p, uint64_t n, const uint32_t* flag) {
+  if (*flag == 0u) return;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
 }  // namespace
 
-hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s) {
+hipError_t fcs_launch_frame_setup(const FrameSetupArgs& a, hipStream_t s, bool frustum) {
   if (!a.nb) return hipSuccess;
-  hipLaunchKernelGGL(frame_setup_kernel, dim3((a.nb + 255) / 256), dim3(256), 0, s, a);
+  if (frustum)
+    hipLaunchKernelGGL(frame_setup_kernel<true>, dim3((a.nb + 255) / 256), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(frame_setup_kernel<false>, dim3((a.nb + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t fcs_launch_miss_enqueue(const MissEnqueueArgs& a, hipStream_t s) {
+  if (!a.nb) return hipSuccess;
+  hipLaunchKernelGGL(miss_enqueue_kernel, dim3((a.nb + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t fcs_launch_queue_settle(const QueueSettleArgs& a, hipStream_t s) {
+  if (!a.cap) return hipSuccess;
+  hipLaunchKernelGGL(queue_settle_kernel, dim3((a.cap + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t fcs_launch_prefetch_enqueue(const PrefetchArgs& a, hipStream_t s) {
+  if (!a.nb) return hipSuccess;
+  const dim3 grid((a.nb + 255) / 256);
+  hipLaunchKernelGGL(zero_u32_kernel_m, dim3(5), dim3(256), 0, s, a.hist, (uint64_t)1028);
+  hipLaunchKernelGGL(prefetch_hist_kernel, grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(prefetch_cut_kernel, dim3(1), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(prefetch_enqueue_kernel, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t fcs_launch_zero_if(uint32_t* p, uint64_t n, const uint32_t* flag, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(zero_if_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, flag);
   return hipGetLastError();
 }
 

@@ -1128,7 +1128,9 @@ template <typename LT, class C>
 __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const Lds<LT> S = make_lds<C, LT>(smem);
-  const uint32_t nreq = a.nreq;
+  // a device-built queue carries its count in device memory (at most the
+  // list's capacity, a.nreq)
+  const uint32_t nreq = a.nreq_dev ? min(*a.nreq_dev, a.nreq) : a.nreq;
   const DecodeReq none{0xFFFFFFFFu, 0u, 0ull};
   const DevBrick nobrick{0ull, 0ull, 0u, 0u, 0u, 0u};
   const uint32_t G = gridDim.x;
@@ -1196,7 +1198,7 @@ __global__ void zero_u32_kernel(uint32_t* p, uint64_t n) {
 }
 
 template <typename LT, class C>
-hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int head) {
+hipError_t occupancy(int* out) {
   constexpr size_t lds = Layout<C, LT>::bytes;
   static_assert(lds <= 160 * 1024, "op decoder LDS budget");
   // resident workgroups per CU (queried once per instantiation)
@@ -1210,11 +1212,21 @@ hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint
     if (e != hipSuccess) return e;
     per_cu = nb > 0 ? nb : 1;
   }
+  *out = per_cu;
+  return hipSuccess;
+}
+
+template <typename LT, class C>
+hipError_t launch_cfg(const OpsArgs<LT>& a, uint32_t num_cu, hipStream_t s, uint32_t* grid_out, int head) {
+  constexpr size_t lds = Layout<C, LT>::bytes;
+  int per_cu = 0;
+  if (hipError_t e = occupancy<LT, C>(&per_cu); e != hipSuccess) return e;
   const uint64_t resident = (uint64_t)per_cu * (num_cu ? num_cu : 1);
   const uint32_t grid = (uint32_t)(a.nreq < resident ? a.nreq : resident);
   if (grid_out) *grid_out = grid;
   OpsArgs<LT> b = a;
-  b.head = head == FCS_OPS_HEAD_ON || (head == FCS_OPS_HEAD_AUTO && a.nreq >= FCS_OPS_HEAD_MIN) ? 1u : 0u;
+  // the head pass needs the request count at launch: never for a device-built queue
+  b.head = !a.nreq_dev && (head == FCS_OPS_HEAD_ON || (head == FCS_OPS_HEAD_AUTO && a.nreq >= FCS_OPS_HEAD_MIN)) ? 1u : 0u;
   if (b.head) {
     constexpr size_t hl = Layout<CfgHead, LT>::bytes;
     hipLaunchKernelGGL(ops_head_kernel<LT>, dim3(a.nreq), dim3(64), hl, s, b);
@@ -1246,5 +1258,15 @@ hipError_t fcs_launch_zero(uint32_t* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename LT>
+hipError_t fcs_ops_prepare(uint32_t L, int split) {
+  int per_cu = 0;
+  if (L == 6) return occupancy<LT, CfgLarge>(&per_cu);
+  if (split == FCS_OPS_4WG) return occupancy<LT, CfgSmall4>(&per_cu);
+  return occupancy<LT, CfgSmall>(&per_cu);
+}
+
+template hipError_t fcs_ops_prepare<uint32_t>(uint32_t, int);
+template hipError_t fcs_ops_prepare<uint64_t>(uint32_t, int);
 template hipError_t fcs_launch_ops<uint32_t>(const OpsArgs<uint32_t>&, uint32_t, hipStream_t, uint32_t*, int, int);
 template hipError_t fcs_launch_ops<uint64_t>(const OpsArgs<uint64_t>&, uint32_t, hipStream_t, uint32_t*, int, int);

@@ -161,6 +161,14 @@ class Volume:
         """Direct-mapped pool: brick i owns slot i (fcs_cache_reserve_direct)."""
         check(lib.fcs_cache_reserve_direct(self.h), self.ctx.h)
 
+    def cache_resident(self):
+        """fcs_cache_resident: (resident bricks ascending, the LOD each holds)."""
+        n = check(lib.fcs_cache_resident(self.h, None, None, 0), self.ctx.h)
+        b = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.uint8)
+        check(lib.fcs_cache_resident(self.h, _ptr(b, c_u32p), _ptr(t, c_u8p), n), self.ctx.h)
+        return b, t
+
     def cache_clear(self):
         check(lib.fcs_cache_clear(self.h), self.ctx.h)
 

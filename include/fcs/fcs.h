@@ -190,7 +190,7 @@ int fcs_cache_clear(fcs_volume* vol);
  * renderLoop.js:502-595): resident bricks in ascending index order with the
  * LOD each holds (its finest level for a pyramid); writes min(cap, n) entries
  * (either pointer nullable) and returns n. */
-int64_t fcs_cache_resident(const fcs_volume* vol, uint32_t* bricks, uint8_t* lods, uint64_t cap);
+int64_t fcs_cache_resident(fcs_volume* vol, uint32_t* bricks, uint8_t* lods, uint64_t cap);
 
 /* Decode n (brick, LOD) requests into the brick cache (one batched launch).
  * Bricks already resident at that LOD are not decoded again.
