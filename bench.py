@@ -551,6 +551,11 @@ def bench_config4(args):
         "fps_learned_prefetch": learned["fps_prefetch"],
         "speedup_vs_plain": learned["speedup"],
         "speedup_visibility_driven_vs_plain": visible["speedup"],
+        # device time of the frame's own work (decode + march; the network's
+        # forward pass and the prefetch decodes run outside it)
+        "device_ms_per_frame": {"plain": learned["device_ms_per_frame_plain"],
+                                "visibility_driven": visible["device_ms_per_frame_prefetch"],
+                                "learned": learned["device_ms_per_frame_prefetch"]},
         "miss_rate": learned["miss_rate"],
         "miss_rate_visibility_driven": visible["miss_rate"],
         "on_demand_decodes": {"plain": learned["on_demand_plain"], "visibility_driven": visible["on_demand_prefetch"],

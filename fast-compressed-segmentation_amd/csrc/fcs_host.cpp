@@ -64,6 +64,9 @@ struct DevFrames {
   DecodeReq* d_aq = nullptr;    // prefetch queue (capacity nb)
   uint32_t* d_aqst = nullptr;
   uint32_t* d_small = nullptr;  // DF_* words, then the prefetch histogram (1028 words)
+  uint32_t* d_needed = nullptr;  // per brick: entered by a ray after its first miss
+  uint32_t* d_pixmiss = nullptr; // per pixel: the ray missed (pix_cap pixels)
+  size_t pix_cap = 0;
   void* d_cam = nullptr;        // MarchCam: camera, look-ahead camera
   void* d_cam_ahead = nullptr;
   uint8_t* d_lod = nullptr;     // fixed LODs
@@ -489,7 +492,8 @@ void free_dev_frames(fcs_volume* v) {
   if (!v->df) return;
   drop_dev_graphs(v);
   DevFrames* F = v->df;
-  void* dev[] = {F->d_q, F->d_qst, F->d_aq, F->d_aqst, F->d_small, F->d_cam, F->d_cam_ahead, F->d_lod};
+  void* dev[] = {F->d_q, F->d_qst, F->d_aq, F->d_aqst, F->d_small, F->d_cam, F->d_cam_ahead, F->d_lod,
+                 F->d_needed, F->d_pixmiss};
   for (void* q : dev)
     if (q) (void)hipFree(q);
   void* host[] = {F->h_cam, F->h_cam_ahead, F->h_lod, F->h_small};
@@ -1688,6 +1692,7 @@ int ensure_dev_frames(fcs_volume* v) {
   HIP_TRY(c, hipMalloc(&F->d_cam, sizeof(MarchCam)));
   HIP_TRY(c, hipMalloc(&F->d_cam_ahead, sizeof(MarchCam)));
   HIP_TRY(c, hipMalloc((void**)&F->d_lod, nb));
+  HIP_TRY(c, hipMalloc((void**)&F->d_needed, nb * 4));
   HIP_TRY(c, hipHostMalloc(&F->h_cam, sizeof(MarchCam), hipHostMallocDefault));
   HIP_TRY(c, hipHostMalloc(&F->h_cam_ahead, sizeof(MarchCam), hipHostMallocDefault));
   HIP_TRY(c, hipHostMalloc((void**)&F->h_lod, nb, hipHostMallocDefault));
@@ -1715,7 +1720,8 @@ int issue_queue_decode(fcs_volume* v, DecodeReq* q, uint32_t* status, uint32_t* 
 }
 
 template <typename LT>
-int dev_march(fcs_volume* v, uint32_t w, uint32_t h, bool rgba, uint32_t* misses, const uint32_t* run_if) {
+int dev_march(fcs_volume* v, uint32_t w, uint32_t h, bool rgba, uint32_t* misses, const uint32_t* run_if,
+              uint32_t* pix_miss, const uint32_t* pix_only, uint32_t* needed) {
   fcs_ctx* c = v->ctx;
   MarchCam zc;
   memset(&zc, 0, sizeof zc);
@@ -1725,6 +1731,9 @@ int dev_march(fcs_volume* v, uint32_t w, uint32_t h, bool rgba, uint32_t* misses
                   w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
   shade_args(v, rgba, a);
   a.run_if = run_if;
+  a.pix_miss = pix_miss;
+  a.pix_only = pix_only;
+  a.needed = needed;
   HIP_TRY(c, fcs_launch_march<LT>(a, c->stream));
   return 0;
 }
@@ -1746,18 +1755,24 @@ int enqueue_dev_frame(fcs_volume* v, int form, uint32_t w, uint32_t h, bool rgba
   if (plain)
     if (int rc = issue_queue_decode(v, F.d_q, F.d_qst, sm + DF_QCOUNT, sm + DF_CLAIM, split, s)) return rc;
   HIP_TRY(c, fcs_launch_zero(v->d_visited, v->nb, s));
-  auto march = [&](uint32_t* misses, const uint32_t* run_if) {
-    return v->label_bits == 32 ? dev_march<uint32_t>(v, w, h, rgba, misses, run_if)
-                               : dev_march<uint64_t>(v, w, h, rgba, misses, run_if);
+  auto march = [&](uint32_t* misses, const uint32_t* run_if, uint32_t* pix_miss, const uint32_t* pix_only,
+                   uint32_t* needed) {
+    return v->label_bits == 32 ? dev_march<uint32_t>(v, w, h, rgba, misses, run_if, pix_miss, pix_only, needed)
+                               : dev_march<uint64_t>(v, w, h, rgba, misses, run_if, pix_miss, pix_only, needed);
   };
-  if (int rc = march(sm + DF_MISS1, nullptr)) return rc;
-  if (!plain) {
-    MissEnqueueArgs ma{sm + DF_MISS1, v->d_visited, v->d_meta, v->d_offs, v->d_res, F.d_q, sm + DF_QCOUNT,
-                       v->nb, v->stride};
+  if (plain) {
+    if (int rc = march(sm + DF_MISS1, nullptr, nullptr, nullptr, nullptr)) return rc;
+  } else {
+    // the first march marks the bricks rays enter up to their first miss as
+    // visited and the rest as needed; the second re-traces only the rays
+    // that missed, adding the bricks they enter to the visited set
+    HIP_TRY(c, fcs_launch_zero(F.d_needed, v->nb, s));
+    if (int rc = march(sm + DF_MISS1, nullptr, F.d_pixmiss, nullptr, F.d_needed)) return rc;
+    MissEnqueueArgs ma{sm + DF_MISS1, v->d_visited, F.d_needed, v->d_meta, v->d_offs, v->d_res, F.d_q,
+                       sm + DF_QCOUNT, v->nb, v->stride};
     HIP_TRY(c, fcs_launch_miss_enqueue(ma, s));
     if (int rc = issue_queue_decode(v, F.d_q, F.d_qst, sm + DF_QCOUNT, sm + DF_CLAIM, split, s)) return rc;
-    HIP_TRY(c, fcs_launch_zero_if(v->d_visited, v->nb, sm + DF_MISS1, s));
-    if (int rc = march(sm + DF_MISS2, sm + DF_MISS1)) return rc;
+    if (int rc = march(sm + DF_MISS2, sm + DF_MISS1, nullptr, F.d_pixmiss, nullptr)) return rc;
   }
   HIP_TRY(c, hipMemcpyAsync(F.h_small, sm, 8 * 4, hipMemcpyDeviceToHost, s));
   return 0;
@@ -1827,6 +1842,17 @@ int dev_frame(fcs_volume* v, const fcs_camera* cam, const uint8_t* brick_lod, ui
     if (int rc = ensure_tf(v)) return rc;
   const size_t npx = (size_t)w * h;
   if (int rc = ensure_frame_buffers(v, npx, rgba)) return rc;
+  if (npx > F.pix_cap) {
+    if (!F.graphs.empty()) {
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      drop_dev_graphs(v);
+    }
+    if (F.d_pixmiss) (void)hipFree(F.d_pixmiss);
+    F.d_pixmiss = nullptr;
+    F.pix_cap = 0;
+    HIP_TRY(c, hipMalloc((void**)&F.d_pixmiss, npx * 4));
+    F.pix_cap = npx;
+  }
   if (int rc = push_residency(v)) return rc;
   memcpy(F.h_cam, cam, sizeof(MarchCam));
   if (brick_lod) memcpy(F.h_lod, brick_lod, v->nb);

@@ -83,6 +83,11 @@ struct MarchArgs {
   float light[3];           // light direction (towards the light, not normalised)
   float ambient, diffuse;
   const uint32_t* run_if;   // nullable: the launch does nothing when *run_if == 0
+  // A march that re-traces only the rays that missed (all nullable):
+  uint32_t* pix_miss;       // per pixel: 1 when the ray entered a non-resident brick
+  const uint32_t* pix_only; // per pixel: trace only where this is non-zero (others keep their outputs)
+  uint32_t* needed;         // per brick: entered by a ray after its first miss (`visited` then holds
+                            // only bricks entered up to a ray's first miss, which a full march enters too)
 };
 
 // Transfer function hash slot of a label (SplitMix64 finaliser).
@@ -129,6 +134,7 @@ struct FrameSetupArgs {
 struct MissEnqueueArgs {
   const uint32_t* misses;
   const uint32_t* visited;
+  const uint32_t* needed;  // nullable: bricks entered after a ray's first miss
   uint32_t* meta;
   uint64_t* offs;
   uint64_t* res_w;

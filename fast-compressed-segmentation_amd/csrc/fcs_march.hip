@@ -176,8 +176,10 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= a.w || py >= a.h) return;
   if (a.run_if && *a.run_if == 0u) return;
-  if (a.cam_dev) a.cam = *a.cam_dev;
   const size_t pix = (size_t)py * a.w + px;
+  if (a.pix_only && a.pix_only[pix] == 0u) return;
+  if (a.cam_dev) a.cam = *a.cam_dev;
+  bool missed = false;  // entered a non-resident brick
   LT hit_label = 0;
   uint64_t hit_vox = ~0ull;
   float hit_t = -1.0f;
@@ -267,7 +269,10 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       const uint32_t bi = fcs_morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
       const uint32_t meta = a.meta[bi];
       if (!(meta & FCS_META_EMPTY)) {
-        if (a.visited) a.visited[bi] = 1u;
+        if (a.visited) {
+          if (missed && a.needed) a.needed[bi] = 1u;
+          else a.visited[bi] = 1u;
+        }
         const int bmin0 = brick[0] * b, bmin1 = brick[1] * b, bmin2 = brick[2] * b;
         const float tEnterBrick = fmx(brickEntryT, tEnter);
         float tExitBrick = fmn(fmn(btmax[0], btmax[1]), btmax[2]);
@@ -313,6 +318,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
         const bool resident = (meta & FCS_META_RESIDENT) != 0;
         if (lod != 0 && !resident) {
           if (a.misses) atomicAdd(a.misses, 1u);
+          missed = true;
         } else {
           const LT* lab = a.cache + a.offs[bi];
           const float lodScale = (float)lodSize / (float)b;
@@ -418,6 +424,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       if (tt > tExit) break;
     }
   }
+  if (a.pix_miss) a.pix_miss[pix] = missed ? 1u : 0u;
   if (a.out_label) a.out_label[pix] = hit_label;
   if (a.out_voxel) a.out_voxel[pix] = hit_vox;
   if (a.out_t) a.out_t[pix] = hit_t;
@@ -545,7 +552,8 @@ __global__ __launch_bounds__(256) void miss_enqueue_kernel(MissEnqueueArgs a) {
   uint32_t m = 0;
   if (i < a.nb) {
     m = a.meta[i];
-    need = a.visited[i] && !(m & (FCS_META_RESIDENT | FCS_META_EMPTY)) && (m & FCS_META_LOD_MASK) > 0;
+    need = (a.visited[i] || (a.needed && a.needed[i])) && !(m & (FCS_META_RESIDENT | FCS_META_EMPTY)) &&
+           (m & FCS_META_LOD_MASK) > 0;
   }
   const uint32_t slot = queue_slot(need, a.qcount);
   if (!need) return;

@@ -54,8 +54,8 @@ def run_path(vol, cams, w, h, mlp=None, threshold=0.5, max_prefetch=2048, ahead=
     used per brick, visited mask and the frame checksum."""
     vol.cache_clear()
     nf = len(cams)
-    out = {"ms": np.zeros(nf), "on_demand": np.zeros(nf, np.int64), "prefetched": [], "checksum": [],
-           "visited": [], "lod": []}
+    out = {"ms": np.zeros(nf), "device_ms": np.zeros(nf), "on_demand": np.zeros(nf, np.int64), "prefetched": [],
+           "checksum": [], "visited": [], "lod": []}
     for i, cam in enumerate(cams):
         t0 = time.perf_counter()
         if mlp is None:
@@ -74,6 +74,7 @@ def run_path(vol, cams, w, h, mlp=None, threshold=0.5, max_prefetch=2048, ahead=
                                          max_prefetch=max_prefetch, want=("brick_visited",))
         cs = vol.frame_checksum()
         out["ms"][i] = (time.perf_counter() - t0) * 1e3
+        out["device_ms"][i] = fr.march_ms + fr.decode_ms
         out["on_demand"][i] = fr.bricks_decoded
         out["prefetched"].append(pf)
         out["checksum"].append(cs)
@@ -103,6 +104,8 @@ def compare(plain_runs, pf_runs):
     prefetches and frames whose checksum differs."""
     t_plain = sum(r["ms"].sum() for r in plain_runs)
     t_pf = sum(r["ms"].sum() for r in pf_runs)
+    d_plain = sum(r["device_ms"].sum() for r in plain_runs)
+    d_pf = sum(r["device_ms"].sum() for r in pf_runs)
     frames = sum(len(r["ms"]) for r in plain_runs)
     od_plain = sum(int(r["on_demand"].sum()) for r in plain_runs)
     od_pf = sum(int(r["on_demand"].sum()) for r in pf_runs)
@@ -118,6 +121,8 @@ def compare(plain_runs, pf_runs):
         "fps_plain": frames / (t_plain * 1e-3),
         "fps_prefetch": frames / (t_pf * 1e-3),
         "speedup": t_plain / t_pf,
+        "device_ms_per_frame_plain": d_plain / frames,
+        "device_ms_per_frame_prefetch": d_pf / frames,
         "on_demand_plain": od_plain,
         "on_demand_prefetch": od_pf,
         "miss_rate": od_pf / max(od_plain, 1),
