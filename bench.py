@@ -152,6 +152,16 @@ def main():
         "ops_kernel": {"ms": ops_ms, "bytes": ops_bytes, "gbs": ops_bytes / (ops_ms * 1e-3) / 1e9},
         "march_kernel": {"ms": march_ms, "bytes": None, "gbs": None},
     }
+    ctx.set_march_stats(True)
+    mb = []
+    for cam in cams:
+        vol.render(cam, W, H, want=())
+        tm = ctx.timing()
+        mb.append(4 * tm["march_samples"] + 48 * tm["march_bricks_entered"] + 32 * tm["march_empty_tested"]
+                  + W * H * 16)
+    ctx.set_march_stats(False)
+    kernels["march_kernel"]["bytes"] = float(np.mean(mb))
+    kernels["march_kernel"]["gbs"] = kernels["march_kernel"]["bytes"] / (march_ms * 1e-3) / 1e9
     dom = max(("rans_kernel", "ops_kernel"), key=lambda k: kernels[k]["ms"])
     if march_ms > kernels[dom]["ms"]:
         pass  # the march is reported per frame; decode kernels carry the roofline
@@ -336,8 +346,19 @@ def bench_config3(args):
             raise RuntimeError("frame needed a decode after the full decode")
         march.append(f.march_ms)
     march_ms = float(np.mean(march))
-
-    # batched requests, checked against the oracle
+    # the march's algorithmic bytes per frame: what the rays read (4 B per
+    # label sample, the 48 B records of each non-empty brick entered, 32 B per
+    # empty brick tested) and the per-pixel outputs (label 4 B, voxel index
+    # 8 B, t 4 B), counted by a separate pass over the same poses
+    ctx.set_march_stats(True)
+    mbytes = []
+    for cam in cams:
+        vol.render(cam, W, H, brick_lod=finest, want=())
+        tm = ctx.timing()
+        mbytes.append(4 * tm["march_samples"] + 48 * tm["march_bricks_entered"] + 32 * tm["march_empty_tested"]
+                      + W * H * 16)
+    ctx.set_march_stats(False)
+    march_bytes = float(np.mean(mbytes))
     from oracle import oracle as O
 
     rng = np.random.default_rng(2)
@@ -395,6 +416,11 @@ def bench_config3(args):
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, profiles/hbm_traffic_2048cube_b32.md"},
         "frames_per_s_1080p": 1000.0 / march_ms, "rays_per_s": W * H * 1000.0 / march_ms,
         "march_ms_per_frame": march_ms,
+        "march_kernel": {"ms": march_ms, "bytes_per_frame": march_bytes,
+                         "gbs": march_bytes / (march_ms * 1e-3) / 1e9,
+                         "frac": march_bytes / (march_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": measured_traffic("hbm_traffic_2048cube_b32.md", "march_kernel")
+                         if gen.size == 2048 else None},
         "rans_gbs": rans_bytes / (rans_ms * 1e-3) / 1e9,
         "rans_autotune": {"mapping": RANS_MAPPINGS.get(tune["rans_mapping"]), "lane_ms": tune["tune_lane_ms"],
                           "wave_table_ms": tune["tune_wave_ms"], "wave_ballot_ms": tune["tune_ballot_ms"],

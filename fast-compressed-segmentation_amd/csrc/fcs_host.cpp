@@ -144,6 +144,7 @@ struct fcs_volume {
   uint32_t* d_visited = nullptr;
   uint32_t* d_misses = nullptr;
   unsigned long long* d_csum = nullptr;
+  unsigned long long* d_mstats = nullptr;  // march statistics slots (3 x FCS_MARCH_STATS_SLOTS)
   size_t frame_cap = 0;
   size_t frame_px = 0;  // pixels of the last rendered frame
   // device frames: d_res is changed by the device and is newer than the
@@ -732,6 +733,11 @@ int fcs_set_option(fcs_ctx* c, int option, int64_t value) {
     c->ops_split = (int)value;
     return 0;
   }
+  if (option == FCS_OPT_MARCH_STATS) {
+    if (value < 0 || value > 1) return fail(c, FCS_E_INVALID, "bad march stats flag");
+    c->march_stats = value != 0;
+    return 0;
+  }
   if (option == FCS_OPT_OPS_HEAD) {
     if (value < FCS_OPS_HEAD_AUTO || value > FCS_OPS_HEAD_OFF) return fail(c, FCS_E_INVALID, "bad head mode");
     c->ops_head = (int)value;
@@ -957,7 +963,7 @@ void fcs_volume_release_device(fcs_volume* v) {
   free_dev_frames(v);
   void* ptrs[] = {v->d_stream, v->d_bricks, v->d_pal, v->d_pal0, v->d_nib, v->d_rres, v->d_counter, v->d_rtab, v->d_rlist,
                   v->d_cache, v->d_meta, v->d_offs, v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited,
-                  v->d_misses, v->d_csum, v->d_res, v->d_tf_keys, v->d_tf_rgba, v->d_frgba};
+                  v->d_misses, v->d_csum, v->d_res, v->d_tf_keys, v->d_tf_rgba, v->d_frgba, v->d_mstats};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   free_lane(v->lane_main);
@@ -1622,6 +1628,10 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
   HIP_TRY(c, hipMemcpyAsync(v->d_offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, fcs_launch_zero(v->d_visited, v->nb, c->stream));
   HIP_TRY(c, fcs_launch_zero(v->d_misses, 1, c->stream));
+  if (c->march_stats) {
+    if (!v->d_mstats) HIP_TRY(c, hipMalloc((void**)&v->d_mstats, 3 * FCS_MARCH_STATS_SLOTS * 8));
+    HIP_TRY(c, fcs_launch_zero(reinterpret_cast<uint32_t*>(v->d_mstats), 6 * FCS_MARCH_STATS_SLOTS, c->stream));
+  }
   const float aspect = (float)((double)w / (double)h);
   MarchCam mc;
   memcpy(&mc, cam, sizeof mc);
@@ -1631,12 +1641,14 @@ int march_pass(fcs_volume* v, const fcs_camera* cam, const std::vector<uint8_t>&
                           (uint32_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
     shade_args(v, rgba, a);
+    a.stats = c->march_stats ? v->d_mstats : nullptr;
     HIP_TRY(c, fcs_launch_march<uint32_t>(a, c->stream));
   } else {
     MarchArgs<uint64_t> a{mc, nullptr, v->d_meta, v->d_offs, (const uint64_t*)v->d_pal0, (const uint64_t*)v->d_cache,
                           (uint64_t*)v->d_flabel, v->d_fvoxel, v->d_ft, v->d_visited, v->d_misses,
                           w, h, (float)w, (float)h, aspect, (float)(v->B * v->b), v->b, v->B};
     shade_args(v, rgba, a);
+    a.stats = c->march_stats ? v->d_mstats : nullptr;
     HIP_TRY(c, fcs_launch_march<uint64_t>(a, c->stream));
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
@@ -1654,7 +1666,20 @@ int finish_frame(fcs_volume* v, const std::vector<uint8_t>& lods, uint32_t w, ui
   if (out->brick_visited)
     HIP_TRY(c, hipMemcpyAsync(out->brick_visited, v->d_visited, (size_t)v->nb * 4, hipMemcpyDeviceToHost, c->stream));
   if (out->brick_lod_used) memcpy(out->brick_lod_used, lods.data(), v->nb);
+  std::vector<unsigned long long> mst;
+  if (c->march_stats && v->d_mstats) {
+    mst.resize(3 * FCS_MARCH_STATS_SLOTS);
+    HIP_TRY(c, hipMemcpyAsync(mst.data(), v->d_mstats, mst.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (!mst.empty()) {
+    uint64_t sum[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < 3; ++k)
+      for (uint32_t j = 0; j < FCS_MARCH_STATS_SLOTS; ++j) sum[k] += mst[k * FCS_MARCH_STATS_SLOTS + j];
+    c->timing.march_samples = sum[0];
+    c->timing.march_bricks_entered = sum[1];
+    c->timing.march_empty_tested = sum[2];
+  }
   v->frame_px = npx;
   if (int rc = settle_prefetch(v)) return rc;
   float ms = 0;

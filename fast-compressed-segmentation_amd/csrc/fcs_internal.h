@@ -39,6 +39,7 @@ struct fcs_ctx {
   int rans_mapping = FCS_RANS_MAPPING_AUTO;
   int ops_split = FCS_OPS_SPLIT_AUTO;
   int ops_head = FCS_OPS_HEAD_AUTO;
+  bool march_stats = false;
   std::vector<fcs_volume*> volumes;
   std::vector<fcs_mlp*> mlps;
 };

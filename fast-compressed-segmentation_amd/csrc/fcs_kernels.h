@@ -88,7 +88,12 @@ struct MarchArgs {
   const uint32_t* pix_only; // per pixel: trace only where this is non-zero (others keep their outputs)
   uint32_t* needed;         // per brick: entered by a ray after its first miss (`visited` then holds
                             // only bricks entered up to a ray's first miss, which a full march enters too)
+  // nullable: what the rays read, summed -- samples, non-empty bricks entered,
+  // empty bricks tested -- each over FCS_MARCH_STATS_SLOTS slots (cleared by
+  // the caller), for the march's algorithmic bytes
+  unsigned long long* stats;
 };
+#define FCS_MARCH_STATS_SLOTS 1024u
 
 // Transfer function hash slot of a label (SplitMix64 finaliser).
 __host__ __device__ __forceinline__ uint32_t fcs_tf_slot(uint64_t label, uint32_t mask) {

@@ -170,7 +170,7 @@ __device__ float shadow_ray(const MarchArgs<LT>& a, const float start[3], const 
   return 1.0f * (1.0f - transmittance) + vis * transmittance;
 }
 
-template <typename LT, bool RGBA>
+template <typename LT, bool RGBA, bool STATS>
 __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   if (a.pix_only && a.pix_only[pix] == 0u) return;
   if (a.cam_dev) a.cam = *a.cam_dev;
   bool missed = false;  // entered a non-resident brick
+  uint32_t n_samples = 0, n_bricks = 0, n_empty = 0;  // STATS: what the ray read
   LT hit_label = 0;
   uint64_t hit_vox = ~0ull;
   float hit_t = -1.0f;
@@ -268,6 +269,10 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       if (brick[0] < 0 || brick[1] < 0 || brick[2] < 0 || brick[0] >= B || brick[1] >= B || brick[2] >= B) break;
       const uint32_t bi = fcs_morton3((uint32_t)brick[0], (uint32_t)brick[1], (uint32_t)brick[2]);
       const uint32_t meta = a.meta[bi];
+      if constexpr (STATS) {
+        if (meta & FCS_META_EMPTY) ++n_empty;
+        else ++n_bricks;
+      }
       if (!(meta & FCS_META_EMPTY)) {
         if (a.visited) {
           if (missed && a.needed) a.needed[bi] = 1u;
@@ -325,6 +330,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
           float tcur = tEnterBrick;
           for (uint32_t s = 0; s < maxs; ++s) {
             if (vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < b && vox[1] < b && vox[2] < b) {
+              if constexpr (STATS) ++n_samples;
               LT v;
               if (lod == 0) {
                 v = a.pal0[bi];
@@ -423,6 +429,13 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       brickEntryT = tt + biasf(tt);
       if (tt > tExit) break;
     }
+  }
+  if constexpr (STATS) {
+    // spread over FCS_MARCH_STATS_SLOTS slots per counter (the host sums them)
+    const uint32_t slot = (uint32_t)(pix % FCS_MARCH_STATS_SLOTS);
+    if (n_samples) atomicAdd(&a.stats[slot], (unsigned long long)n_samples);
+    if (n_bricks) atomicAdd(&a.stats[FCS_MARCH_STATS_SLOTS + slot], (unsigned long long)n_bricks);
+    if (n_empty) atomicAdd(&a.stats[2 * FCS_MARCH_STATS_SLOTS + slot], (unsigned long long)n_empty);
   }
   if (a.pix_miss) a.pix_miss[pix] = missed ? 1u : 0u;
   if (a.out_label) a.out_label[pix] = hit_label;
@@ -689,9 +702,11 @@ hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {
   const dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
   if (a.out_rgba) {
     if (!a.tf_keys || !a.tf_rgba) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((march_kernel<LT, true>), grid, dim3(256), 0, s, a);
+    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, true, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((march_kernel<LT, true, false>), grid, dim3(256), 0, s, a);
   } else {
-    hipLaunchKernelGGL((march_kernel<LT, false>), grid, dim3(256), 0, s, a);
+    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, false, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((march_kernel<LT, false, false>), grid, dim3(256), 0, s, a);
   }
   return hipGetLastError();
 }

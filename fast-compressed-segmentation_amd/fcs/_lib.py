@@ -115,6 +115,9 @@ class TimingStruct(ctypes.Structure):
         ("tune_win_ms", ctypes.c_float),
         ("tune_win_ballot_ms", ctypes.c_float),
         ("pipeline_ms", ctypes.c_float),
+        ("march_samples", ctypes.c_uint64),
+        ("march_bricks_entered", ctypes.c_uint64),
+        ("march_empty_tested", ctypes.c_uint64),
     ]
 
 

@@ -109,6 +109,11 @@ class Context:
         least 1024 requests, 1 = always, 2 = never."""
         check(lib.fcs_set_option(self.h, 3, int(mode)), self.h)
 
+    def set_march_stats(self, on):
+        """Host-built frames count what their rays read (timing march_samples,
+        march_bricks_entered, march_empty_tested)."""
+        check(lib.fcs_set_option(self.h, 4, 1 if on else 0), self.h)
+
     def sync(self):
         check(lib.fcs_sync(self.h), self.h)
 

@@ -143,6 +143,12 @@ typedef struct {
   float tune_win_ms;      /* autotuner: whole-volume rANS time, one wave per brick, scalar byte window (table) */
   float tune_win_ballot_ms; /* autotuner: same, ballot over a per-lane CDF */
   float pipeline_ms;      /* last fcs_decode_volume: both stages, overlapped */
+  /* last fcs_render / fcs_render_visible march with FCS_OPT_MARCH_STATS set
+   * (host-built frames): what its rays read -- label samples, non-empty
+   * bricks entered (each reads its 48-byte records), empty bricks tested */
+  uint64_t march_samples;
+  uint64_t march_bricks_entered;
+  uint64_t march_empty_tested;
 } fcs_timing;
 
 /* Options (fcs_set_option). */
@@ -163,6 +169,7 @@ typedef struct {
                                >= 1024 requests */
 #define FCS_OPS_HEAD_ON 1   /* always use that pass */
 #define FCS_OPS_HEAD_OFF 2  /* never: the op decoder's first wave decodes those levels */
+#define FCS_OPT_MARCH_STATS 4 /* 1: host-built frames count what their rays read (fcs_timing.march_*); 0: off */
 
 int fcs_create(int hip_device, fcs_ctx** out);
 void fcs_destroy(fcs_ctx* ctx);

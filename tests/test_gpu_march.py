@@ -129,3 +129,50 @@ def test_device_frames_against_exact_traversal(cfg2):
         for key in tot:
             tot[key] += st[key]
     assert tot["grazing"] <= 1e-3 * tot["pixels"], tot
+
+
+def test_march_stats_match_column_walk(cfg1):
+    """The march's read counts (samples, non-empty bricks entered, empty
+    bricks tested) on an orthographic +z frame at the finest LOD equal a
+    column walk: a ray tests the bricks of its column front to back until
+    the brick of its first hit, samples every voxel of the non-empty ones up
+    to the hit. The counts do not change the frame."""
+    vol, ds, pyr = cfg1
+    ctx = vol.ctx
+    cam = fcs.Camera.ortho_plus_z(128)
+    lods = np.full(64, 5, np.uint8)
+    plain = vol.render(cam, 128, 128, brick_lod=lods)
+    ctx.set_march_stats(True)
+    try:
+        f = vol.render(cam, 128, 128, brick_lod=lods)
+        tm = ctx.timing()
+    finally:
+        ctx.set_march_stats(False)
+    assert np.array_equal(f.label, plain.label) and np.array_equal(f.voxel, plain.voxel)
+    vol_lab = np.zeros((128, 128, 128), np.uint64)  # z, y, x
+    from conftest import c1
+    idx = np.arange(32 ** 3)
+    lx, ly, lz = c1(idx), c1(idx >> 1), c1(idx >> 2)
+    empty = [bool(np.all(ds.palette(i) == 0)) for i in range(64)]
+    for i in range(64):
+        bx, by, bz = int(c1(i)), int(c1(i >> 1)), int(c1(i >> 2))
+        vol_lab[bz * 32 + lz, by * 32 + ly, bx * 32 + lx] = pyr[i][O.pyramid_offset(5):]
+    samples = bricks = tested_empty = 0
+    for y in range(128):
+        for x in range(128):
+            for bz in range(4):
+                bi = int(sum(((x // 32 >> k) & 1) << (3 * k) | ((y // 32 >> k) & 1) << (3 * k + 1) |
+                             ((bz >> k) & 1) << (3 * k + 2) for k in range(3)))
+                if empty[bi]:
+                    tested_empty += 1
+                    continue
+                bricks += 1
+                col = vol_lab[bz * 32:(bz + 1) * 32, y, x]
+                nz = np.nonzero(col)[0]
+                if len(nz):
+                    samples += int(nz[0]) + 1
+                    break
+                samples += 32
+    print(f"[gpu] march stats {tm['march_samples']} {tm['march_bricks_entered']} {tm['march_empty_tested']} "
+          f"column walk {samples} {bricks} {tested_empty}", flush=True)
+    assert (tm["march_samples"], tm["march_bricks_entered"], tm["march_empty_tested"]) == (samples, bricks, tested_empty)
