@@ -16,6 +16,7 @@ import json
 import os
 import struct
 import sys
+import tempfile
 
 import numpy as np
 
@@ -28,7 +29,8 @@ import fcs  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 SHIPPED = "/root/reference/encoder/CSVEncoder/compressed_dataset.csbd"
-TMP = os.path.join(ROOT, "_work", "golden_tmp.csbd")
+# scratch .csbd the reference encoder writes and loads back; set by main()
+TMP = None
 
 
 def h(a):
@@ -129,7 +131,13 @@ def shipped_subset():
 
 
 def main():
-    os.makedirs(os.path.dirname(TMP), exist_ok=True)
+    global TMP
+    with tempfile.TemporaryDirectory(prefix="fcs_golden_") as d:
+        TMP = os.path.join(d, "golden_tmp.csbd")
+        _write_all()
+
+
+def _write_all():
     manifest = {}
     for name, (vol, b) in case_volumes().items():
         data, pyr = O.ref_encode(vol, b, TMP, with_pyramid=True)
