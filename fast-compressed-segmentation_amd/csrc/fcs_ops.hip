@@ -376,26 +376,34 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
     const uint32_t g0 = wi * 8;
     // trailing run of 12/13 nibbles over this lane's valid nibbles, the op /
     // palette-advance counts for both entering parities, and which nibbles
-    // are ops (bit j) for both parities
-    uint32_t tr = 0, cnt0 = 0, cnt1 = 0, iso0 = 0, iso1 = 0;
-    if (__ballot(inw) != 0) {
-      uint32_t pd0 = 0, pd1 = 1;
-#pragma unroll
-      for (uint32_t j = 0; j < 8; ++j) {
-        const uint32_t g = g0 + j;
-        const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
-        const bool valid = inw && g >= ns && g < ne;
-        tr = (valid && op == FCS_OP_PD) ? tr + 1 : 0;
-        if (g == ns) { pd0 = 0; pd1 = 0; }
-        const uint32_t inc = 1u | ((op == FCS_OP_PA) ? 0x10000u : 0u);
-        const bool o0 = valid && !pd0, o1 = valid && !pd1;
-        cnt0 += o0 ? inc : 0u;
-        cnt1 += o1 ? inc : 0u;
-        iso0 |= o0 ? (1u << j) : 0u;
-        iso1 |= o1 ? (1u << j) : 0u;
-        pd0 = (o0 && op == FCS_OP_PD) ? 1u : 0u;
-        pd1 = (o1 && op == FCS_OP_PD) ? 1u : 0u;
-      }
+    // are ops (flag at bit 4j) for both parities. Bit-parallel over the
+    // word's 8 nibbles: a nibble is an op unless the one before it is a Pd
+    // op (whose delta it then is); so within a run of Pd nibbles the ops
+    // alternate from the run's first nibble, or from the one after it when
+    // the run starts the word and the lane enters after a Pd op (parity 1).
+    // A level's first nibble enters with parity 0.
+    uint32_t tr, cnt0, cnt1, iso0, iso1;
+    {
+      constexpr uint32_t FL = 0x11111111u, EV = 0x01010101u, OD = 0x10101010u;
+      const int lo = (int)ns - (int)g0, hi = (int)ne - (int)g0;
+      const uint32_t mlo = (lo <= 0) ? ~0u : (lo >= 8) ? 0u : (~0u << (4 * lo));
+      const uint32_t mhi = (hi >= 8) ? ~0u : (hi <= 0) ? 0u : ((1u << (4 * hi)) - 1u);
+      const uint32_t V = inw ? (FL & mlo & mhi) : 0u;  // valid nibbles of this level
+      const uint32_t b3 = word >> 3, b2 = word >> 2, b1 = word >> 1;
+      const uint32_t Q = b3 & b2 & ~b1 & V;   // Pd (12/13)
+      const uint32_t Pa = b3 & ~b2 & ~b1 & V;  // Pa (8/9)
+      const uint32_t Qf = Q * 15u;             // runs of Pd nibbles as runs of bits
+      const uint32_t S0 = Q & ~(Q << 4);       // run starts
+      const uint32_t pin = (g0 > ns) ? 1u : 0u;
+      const uint32_t S1 = S0 & ~pin;
+      // runs that start on an even nibble: adding 1 at the start clears the run
+      const uint32_t r0 = Qf & ~(Qf + (S0 & EV)), r1 = Qf & ~(Qf + (S1 & EV));
+      const uint32_t d0 = Q & ((r0 & EV) | (~r0 & OD)), d1 = Q & ((r1 & EV) | (~r1 & OD));  // Pd ops
+      iso0 = V & ~(d0 << 4);
+      iso1 = V & ~((d1 << 4) | pin);
+      cnt0 = (uint32_t)__popc(iso0) | ((uint32_t)__popc(iso0 & Pa) << 16);
+      cnt1 = (uint32_t)__popc(iso1) | ((uint32_t)__popc(iso1 & Pa) << 16);
+      tr = (uint32_t)__clz(~Qf) >> 2;  // 8 when every nibble is a valid Pd
     }
     const uint64_t Pm = __ballot(tr & 1u);
     const uint64_t Nm = __ballot(tr != 8);
@@ -451,11 +459,25 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         pv1 = io.vcur[m1];
       }
       bool e_pal = false, e_ovr = false, e_op = false;
+      if constexpr (MODE == MODE_COMPACT) {
+        // the op nibble as it is, tagged with bit 15 (palette results, all
+        // below 0x8000, overwrite their slots below); resolve_compact turns
+        // it into a value or a node reference once the level is parsed
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+          const uint32_t v = (word >> (4 * j)) & 0xFu;
+          const bool isop = (iso >> (4 * j)) & 1u;
+          const bool live = isop && k < K;
+          e_op = e_op || (live && (v >> 1) > FCS_OP_PD);
+          *(live ? &io.opv[k] : dum16) = (uint16_t)(FCS_REF16 | v);
+          k += isop ? 1u : 0u;
+        }
+      } else {
 #pragma unroll
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t v = (word >> (4 * j)) & 0xFu;
         const uint32_t op = v >> 1;
-        const bool isop = (iso >> j) & 1u;
+        const bool isop = (iso >> (4 * j)) & 1u;
         const bool live = isop && k < K;
         const bool second = (k >> 3) != q0;
         const uint32_t m = second ? m1 : m0, pv = second ? pv1 : pv0;
@@ -484,20 +506,16 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         if constexpr (MODE == MODE_LEVEL) {
           *(live ? &io.vnext[n] : dum16) = (uint16_t)val;
           atomicOr(live ? &io.snext[n >> 5] : dum32, (live && (v & 1u)) ? (1u << (n & 31)) : 0u);
-        } else if constexpr (MODE == MODE_COMPACT) {
-          *(live ? &io.opv[k] : dum16) = (uint16_t)val;
         } else {
           if (live) io.olev[n] = (LT)val;
         }
         k += isop ? 1u : 0u;
       }
+      }
       {
         // Pa / P0 / Pd. Nibble masks with the flag in bit 4j: op nibbles,
         // Pa ops (nibble 8|9), all palette ops (nibbles 8..13)
-        uint32_t isoN = iso & 0xFFu;
-        isoN = (isoN | (isoN << 12)) & 0x000F000Fu;
-        isoN = (isoN | (isoN << 6)) & 0x03030303u;
-        isoN = (isoN | (isoN << 3)) & 0x11111111u;
+        const uint32_t isoN = iso;
         const uint32_t b3 = word >> 3, b2 = word >> 2, b1 = word >> 1;
         const uint32_t paN = b3 & ~b2 & ~b1 & isoN;
         uint32_t pm = b3 & ~(b2 & b1) & isoN;
@@ -534,7 +552,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         // (and its delta); the palette cursor there
         uint32_t kq = k0, ipq = ip_start;
         for (uint32_t j = 0; j < 8; ++j) {
-          if (!((iso >> j) & 1u)) continue;
+          if (!((iso >> (4 * j)) & 1u)) continue;
           const uint32_t op = (word >> (4 * j + 1)) & 0x7u;
           ipq += (op == FCS_OP_PA) ? 1u : 0u;
           if (kq == K - 1) {
@@ -554,6 +572,73 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
   return op_base;
 }
 
+
+// Compact op results of the final level, one active parent (8 children, 8
+// consecutive ops) per step: the op nibbles left by parse_level (bit 15 set)
+// become values or node references, palette results stay. Rx/Ry/Rz name the
+// neighbour across the face of the child's own 2x2x2 block: children with the
+// axis bit clear reference the matching child of the parent's previous
+// neighbour along that axis, children with the bit set take the value of the
+// parent's next neighbour (that level is not decoded yet). Outside the brick
+// the reference reads the parent's output slot, which child 0 has already
+// overwritten when c > 0. The neighbours are worked out once per parent; a
+// reference into a stopped parent takes that parent's value at once (its
+// children all copy it), and one to child 0 takes child 0's result.
+template <typename LT, class C>
+__device__ __forceinline__ void resolve_compact(const Lds<LT>& S, uint16_t* opv, const uint16_t* vcur,
+                                                const uint32_t* scur, uint32_t A, uint32_t l) {
+  const uint32_t pmask = (1u << (3 * l)) - 1u;  // Morton codes of the parent level
+  for (uint32_t gp = threadIdx.x; gp < A; gp += C::NT) {
+    const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
+    if (!((q.x | q.y | q.z | q.w) & 0x80008000u)) continue;  // palette results only
+    uint32_t m, pv;
+    if constexpr (C::PACKED) {
+      const uint32_t e = S.act32[gp];
+      m = e & 0xFFFFu;
+      pv = e >> 16;
+    } else {
+      m = S.act16[gp];
+      pv = vcur[m];
+    }
+    uint32_t prev[3], next[3];
+    bool first[3], last[3];
+#pragma unroll
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+      // (the neighbours wrap inside the level at the faces, where they are unused)
+      const uint32_t mA = (0x09249249u << ax) & pmask;
+      const uint32_t xa = m & mA, rest = m & ~mA;
+      const uint32_t mp = ((xa - 1u) & mA) | rest;
+      first[ax] = xa == 0u;
+      last[ax] = xa == mA;
+      const bool pstop = (scur[mp >> 5] >> (mp & 31)) & 1u;
+      const uint32_t pval = vcur[mp];
+      prev[ax] = pstop ? pval : (FCS_REF16 | (8u * mp));
+      next[ax] = vcur[(((xa | ~mA) + 1u) & mA) | rest];
+    }
+    uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    uint32_t v0 = pv;  // child 0's result
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint32_t sh = 16 * (c & 1);
+      const uint32_t x = (w[c >> 1] >> sh) & 0xFFFFu;
+      const uint32_t oval = (c == 0) ? pv : v0;
+      uint32_t cand[3];
+#pragma unroll
+      for (uint32_t ax = 0; ax < 3; ++ax) {
+        const uint32_t pb = (prev[ax] & FCS_REF16) ? (prev[ax] | (c | (1u << ax))) : prev[ax];
+        cand[ax] = ((c >> ax) & 1u) ? (last[ax] ? oval : next[ax]) : (first[ax] ? oval : pb);
+      }
+      const uint32_t op = (x >> 1) & 0x7u;
+      const uint32_t lo = (op & 1u) ? cand[0] : pv;       // Rx : Rp
+      const uint32_t hi = (op & 1u) ? cand[2] : cand[1];  // Rz : Ry
+      const uint32_t r = (op & 4u) ? 0u : ((op & 2u) ? hi : lo);  // corrupt op nibbles: 0
+      const uint32_t val = (x & FCS_REF16) ? r : x;
+      if (c == 0) v0 = val;
+      w[c >> 1] = (w[c >> 1] & ~(0xFFFFu << sh)) | (val << sh);
+    }
+    *reinterpret_cast<uint4*>(opv + 8 * gp) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
 
 // Per-request state carried from level to level.
 template <typename LT>
@@ -637,6 +722,12 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   v.ns = S.scr[SCR_NS];
   v.ip0 = S.scr[SCR_IP];
   if (l + 1 == c.L && v.ns != nsym) status |= FCS_ST_NSYM_MISMATCH;  // nibbles after the finest level
+  if constexpr (!ONE) {
+    if (mode == MODE_COMPACT) {
+      resolve_compact<LT, C>(S, opv, vcur, scur, A, l);
+      group_sync<CS>();
+    }
+  }
 
   // ---- P2: chase references. A valid reference points to a smaller node,
   // so a chain strictly decreases; anything else is corrupt input. Only op
@@ -704,8 +795,9 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
                 if (nb >= cur) { status |= FCS_ST_NSYM_MISMATCH; x = 0; break; }
                 const uint32_t p = nb >> 3;
                 const uint32_t sw = scur[p >> 5];
+                const uint32_t wp = S.wpre[p >> 5];
                 if ((sw >> (p & 31)) & 1u) { x = vcur[p]; break; }
-                const uint32_t rk = S.wpre[p >> 5] + __popc(~sw & ((1u << (p & 31)) - 1u));
+                const uint32_t rk = wp + __popc(~sw & ((1u << (p & 31)) - 1u));
                 x = opv[8 * rk + (nb & 7u)];
                 if (!(x & FCS_REF16)) break;
                 cur = nb;
