@@ -616,20 +616,46 @@ __global__ __launch_bounds__(256) void prefetch_hist_kernel(PrefetchArgs a) {
 
 // One wave: the lowest bucket still taken whole (hist[1024]) and how many of
 // the bucket below it fit (hist[1025]), from the most probable bucket down.
-__global__ void prefetch_cut_kernel(PrefetchArgs a) {
-  if (threadIdx.x != 0) return;
-  uint32_t acc = 0, cut = 0, part = 0;
-  for (int k = 1023; k >= 0; --k) {
-    const uint32_t c = a.hist[k];
-    if (acc + c > a.cap) {
-      cut = (uint32_t)k + 1u;
-      part = a.cap - acc;
-      break;
-    }
-    acc += c;
+// Lane l holds buckets 1023 - 16 l - j (j = 0..15); a wave scan gives each
+// lane the count above its first bucket, and the first lane whose buckets
+// cross the cap writes the cut.
+__global__ __launch_bounds__(64) void prefetch_cut_kernel(PrefetchArgs a) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t c[16], tot = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 16; ++j) {
+    c[j] = a.hist[1023u - (16u * lane + j)];
+    tot += c[j];
   }
-  a.hist[1024] = cut;
-  a.hist[1025] = part;
+  uint32_t incl = tot;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  uint32_t acc = incl - tot;
+  int found = -1;
+  uint32_t part = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 16; ++j) {
+    if (found < 0 && acc + c[j] > a.cap) {
+      found = (int)j;
+      part = a.cap - acc;
+    }
+    acc += c[j];
+  }
+  const uint64_t m = __ballot(found >= 0);
+  if (m == 0) {
+    if (lane == 0) {
+      a.hist[1024] = 0u;
+      a.hist[1025] = 0u;
+    }
+    return;
+  }
+  if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) {
+    a.hist[1024] = 1024u - (16u * lane + (uint32_t)found);
+    a.hist[1025] = part;
+  }
 }
 
 __global__ __launch_bounds__(256) void prefetch_enqueue_kernel(PrefetchArgs a) {
@@ -721,20 +747,39 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// Order-independent 64-bit sum over the pixels. A fixed grid strides over
+// the frame (four pixels per thread per round, loads issued together), waves
+// reduce with lane shuffles and each workgroup adds its sum once: a few
+// hundred atomics on the one output word instead of one per 256 pixels, which
+// serialised at the cache holding it.
+constexpr uint32_t CSUM_GRID = 512;
+
 template <typename LT>
-__global__ void frame_checksum_kernel(const LT* __restrict__ label, const uint64_t* __restrict__ voxel, uint64_t n,
-                                      unsigned long long* out) {
-  __shared__ unsigned long long red[256];
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void frame_checksum_kernel(const LT* __restrict__ label,
+                                                             const uint64_t* __restrict__ voxel, uint64_t n,
+                                                             unsigned long long* out) {
+  __shared__ unsigned long long red[4];
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
   unsigned long long h = 0;
-  if (i < n) h = mix64(mix64((uint64_t)label[i] + 0x9E3779B97F4A7C15ull * (i + 1)) ^ voxel[i]);
-  red[threadIdx.x] = h;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    uint64_t lab[4], vox[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t i = i0 + k * stride;
+      lab[k] = i < n ? (uint64_t)label[i] : 0ull;
+      vox[k] = i < n ? voxel[i] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t i = i0 + k * stride;
+      if (i < n) h += mix64(mix64(lab[k] + 0x9E3779B97F4A7C15ull * (i + 1)) ^ vox[k]);
+    }
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
 }
 }  // namespace
 
@@ -742,8 +787,9 @@ template <typename LT>
 hipError_t fcs_launch_frame_checksum(const LT* label, const uint64_t* voxel, uint64_t n, unsigned long long* out,
                                      hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(frame_checksum_kernel<LT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, label, voxel, n,
-                     out);
+  const uint64_t groups = (n + 255) / 256;
+  hipLaunchKernelGGL(frame_checksum_kernel<LT>, dim3((unsigned)(groups < CSUM_GRID ? groups : CSUM_GRID)), dim3(256), 0, s,
+                     label, voxel, n, out);
   return hipGetLastError();
 }
 

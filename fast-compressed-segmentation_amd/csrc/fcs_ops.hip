@@ -1206,7 +1206,9 @@ __global__ __launch_bounds__(64) void ops_head_kernel(OpsArgs<LT> a) {
 // the rest is claimed through one atomic counter to even out the finishing
 // times. (One counter shared by every workgroup serialises: claiming every
 // request through it caps a launch of cheap requests at the counter's atomic
-// rate.) Claims map to requests alternately from the heavy front and the light
+// rate.) A launch of at most one request per workgroup -- a frame's few
+// decodes, or none -- is dealt entirely, so its workgroups never touch the
+// counter; one of fewer than two per workgroup deals the first round. Claims map to requests alternately from the heavy front and the light
 // back of the list, one grid-sized chunk at a time, so every workgroup mixes
 // compute-bound requests with cheap ones -- constant bricks are pure label
 // stores, and all of them together at the end would leave the launch's tail
@@ -1226,7 +1228,7 @@ __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   const DecodeReq none{0xFFFFFFFFu, 0u, 0ull};
   const DevBrick nobrick{0ull, 0ull, 0u, 0u, 0u, 0u};
   const uint32_t G = gridDim.x;
-  const uint32_t n_static = (nreq / 2) / G * G;
+  const uint32_t n_static = nreq <= G ? nreq : max((nreq / 2) / G * G, G);
   uint32_t sj = 0;  // thread 0: round-robin requests taken so far
   // claim c -> request: chunk c / G even takes the next G requests from the
   // front, odd from the back (a bijection on [0, nreq))
@@ -1238,6 +1240,7 @@ __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   auto next_index = [&]() -> uint32_t {
     const uint32_t rr = blockIdx.x + sj * G;
     if (rr < n_static) { ++sj; return order(rr); }
+    if (n_static >= nreq) return nreq;  // everything was dealt
     return order(n_static + atomicAdd(a.counter, 1u));
   };
   auto fetch_req = [&](uint32_t i) -> DecodeReq { return i < nreq ? a.reqs[i] : none; };

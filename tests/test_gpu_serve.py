@@ -175,11 +175,18 @@ def test_serve_loop_frames_equal_single_frames(gpu_ctx, vol256):
     assert np.all(res["misses"] == 0)
     assert np.all(res["issue_s"] >= meta["arrival_s"] - 1e-9)
     assert np.all(res["done_s"] >= res["issue_s"])
-    assert len(set(res["view"].tolist())) == 4 or n < 8
+    assert np.all(res["view"] < 4)
     want = _reference_checksums(vols, reqs, n)
     assert np.array_equal(res["checksum"], want)
     st = serve.latency_stats(meta, res)
     print(f"[gpu] p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms", flush=True)
+    # the same requests released at once: every view takes frames (how many
+    # an open-loop trace keeps busy depends on arrival gaps vs frame time)
+    for r in reqs:
+        r.arrival_s = 0.0
+    burst = serve.run(views, vols, reqs)
+    assert len(set(burst["view"].tolist())) == 4
+    assert np.array_equal(burst["checksum"], want)
     for v in views:
         v.close()
     vol_a.close()
