@@ -13,6 +13,8 @@ namespace {
 // A zero direction component has tMax = tDelta = +inf and no slab test.
 
 __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+// x / 2^s with C's truncation toward zero (brick sizes and LOD strides are powers of two)
+__device__ __forceinline__ int tdiv2(int x, int s) { return x >= 0 ? (x >> s) : -((-x) >> s); }
 __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float biasf(float t) { return fmx(t * 1e-6f, 1e-5f); }
 
@@ -30,6 +32,22 @@ __device__ __forceinline__ float advance3(int& v0, int& v1, int& v2, float& m0, 
   v0 += ax ? s0 : 0;
   v1 += ay ? s1 : 0;
   v2 += az ? s2 : 0;
+  return t;
+}
+
+// advance3 that also steps the Morton code `mc` of the cell by one along the
+// chosen axis (dilated-integer increment or decrement under the axis mask M0,
+// M1 or M2; a cell stepping out of the brick wraps, and is never read)
+__device__ __forceinline__ float advance3m(int& v0, int& v1, int& v2, float& m0, float& m1, float& m2, float d0,
+                                           float d1, float d2, int s0, int s1, int s2, uint32_t& mc, uint32_t M0,
+                                           uint32_t M1, uint32_t M2) {
+  const bool ax = m0 < m1 && m0 < m2;
+  const bool ay = !ax && m1 < m2;
+  const float t = advance3(v0, v1, v2, m0, m1, m2, d0, d1, d2, s0, s1, s2);
+  const uint32_t M = ax ? M0 : (ay ? M1 : M2);
+  const int sg = ax ? s0 : (ay ? s1 : s2);
+  const uint32_t u = sg > 0 ? ((mc | ~M) + 1u) : ((mc & M) - 1u);
+  mc = (u & M) | (mc & ~M);
   return t;
 }
 
@@ -186,6 +204,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
   float hit_t = -1.0f;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;  // composited colour (RGBA form)
   const int b = (int)a.brick_size;
+  const int lb = __builtin_ctz(a.brick_size);  // log2 of the brick size
   const int B = (int)a.bricks_per_axis;
   const float G = a.grid;
 
@@ -250,7 +269,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
       const float ts = tEnter + biasf(tEnter);
       _Pragma("unroll") for (int k = 0; k < 3; ++k) {
         const float pos = o[k] + d[k] * ts;
-        brick[k] = (int)floorf(pos) / b;
+        brick[k] = tdiv2((int)floorf(pos), lb);
         step[k] = d[k] > 0.0f ? 1 : (d[k] < 0.0f ? -1 : 0);
         const float bf = (float)brick[k] * (float)b;
         const float nb = d[k] > 0.0f ? bf + (float)b : bf;
@@ -260,8 +279,8 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
     }
     uint32_t maxb = 6;
     _Pragma("unroll") for (int k = 0; k < 3; ++k) {
-      const int be = (int)(o[k] + d[k] * tEnter) / b;
-      const int bx = (int)(o[k] + d[k] * tExit) / b;
+      const int be = tdiv2((int)(o[k] + d[k] * tEnter), lb);
+      const int bx = tdiv2((int)(o[k] + d[k] * tExit), lb);
       maxb += (uint32_t)abs(bx - be);
     }
     float brickEntryT = tEnter;
@@ -295,12 +314,14 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
         }
         const uint32_t lod = meta & FCS_META_LOD_MASK;
         const int lodSize = 1 << lod;
-        const int stride = b / lodSize;
+        const int ls = lb - (int)lod;  // log2 of the LOD stride (lod <= log2 b)
+        const int stride = 1 << ls;
+        const float inv_stride = 1.0f / (float)stride;  // exact: x / stride == x * inv_stride
         _Pragma("unroll") for (int k = 0; k < 3; ++k) vox[k] -= bmin[k];
         int vstep[3] = {step[0], step[1], step[2]};
         if (stride > 1) {
           _Pragma("unroll") for (int k = 0; k < 3; ++k) {
-            int lvx = (vox[k] / stride) * stride;
+            int lvx = tdiv2(vox[k], ls) * stride;
             lvx = lvx < 0 ? 0 : (lvx > b - 1 ? b - 1 : lvx);
             vox[k] = lvx;
             vstep[k] = step[k] * stride;  // one LOD cell per step
@@ -316,8 +337,8 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
         _Pragma("unroll") for (int k = 0; k < 3; ++k) {
           const float ve = o[k] + d[k] * tEnterBrick;
           const float vx = o[k] + d[k] * tExitBrick;
-          const float le = (ve - (float)bmin[k]) / (float)stride;
-          const float lx = (vx - (float)bmin[k]) / (float)stride;
+          const float le = (ve - (float)bmin[k]) * inv_stride;
+          const float lx = (vx - (float)bmin[k]) * inv_stride;
           maxs += (uint32_t)abs((int)ceilf(lx) - (int)floorf(le));
         }
         const bool resident = (meta & FCS_META_RESIDENT) != 0;
@@ -326,8 +347,12 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
           missed = true;
         } else {
           const LT* lab = a.cache + a.offs[bi];
-          const float lodScale = (float)lodSize / (float)b;
           float tcur = tEnterBrick;
+          const uint32_t cm = (1u << (3 * lod)) - 1u;  // Morton codes of the brick's cells
+          const uint32_t M0 = 0x09249249u & cm, M1 = (0x09249249u << 1) & cm, M2 = (0x09249249u << 2) & cm;
+          // the LOD cell of an in-brick voxel: vox * lodSize / b (exact in FP32,
+          // within [0, lodSize)) is vox >> log2(stride); stepped along with the DDA
+          uint32_t mcur = fcs_morton3((uint32_t)vox[0] >> ls, (uint32_t)vox[1] >> ls, (uint32_t)vox[2] >> ls);
           for (uint32_t s = 0; s < maxs; ++s) {
             if (vox[0] >= 0 && vox[1] >= 0 && vox[2] >= 0 && vox[0] < b && vox[1] < b && vox[2] < b) {
               if constexpr (STATS) ++n_samples;
@@ -335,13 +360,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
               if (lod == 0) {
                 v = a.pal0[bi];
               } else {
-                uint32_t cc[3];
-                _Pragma("unroll") for (int k = 0; k < 3; ++k) {
-                  float sc = (float)vox[k] * lodScale;
-                  sc = sc < 0.0f ? 0.0f : (sc > (float)(lodSize - 1) ? (float)(lodSize - 1) : sc);
-                  cc[k] = (uint32_t)sc;
-                }
-                v = lab[fcs_morton3(cc[0], cc[1], cc[2])];
+                v = lab[mcur];
               }
               if (v != 0 && hit_t < 0.0f) {
                 hit_label = v;
@@ -415,8 +434,8 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
                 }
               }
             }
-            const float tt = advance3(vox[0], vox[1], vox[2], vtmax[0], vtmax[1], vtmax[2], vtdelta[0],
-                                      vtdelta[1], vtdelta[2], vstep[0], vstep[1], vstep[2]);
+            const float tt = advance3m(vox[0], vox[1], vox[2], vtmax[0], vtmax[1], vtmax[2], vtdelta[0],
+                                       vtdelta[1], vtdelta[2], vstep[0], vstep[1], vstep[2], mcur, M0, M1, M2);
             tcur = tt;
             if (tt > tExitBrick) break;
           }
