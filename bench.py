@@ -8,7 +8,10 @@ brick at its finest LOD (8.59 G labels), the two stages overlapped
 (fcs_decode_volume). value = decoded Gvoxels/s (HIP events on the library
 stream). Every label of the last step is then compared with the source
 volume, a batched launch of 100k (brick, LOD) requests is timed and compared
-with the oracle, and the march over the decoded volume is timed at 1080p.
+with the oracle, the march over the decoded volume is timed at 1080p, and the
+1024^3 fly-throughs of --config 4 are run for the metric's chosen-vs-plain
+frames/s (learned prefetch against the plain on-demand form), which the line
+carries as `chosen_vs_plain_1080p`.
 
 Other workloads: --config 2 (512^3 demo scale, 64^3 bricks: full pyramid
 decode + 8 orbit poses at 1080p), --config 4 (1024^3 fly-throughs, learned
@@ -58,6 +61,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-config4", action="store_true",
+                    help="config 3: leave out the learned-prefetch fly-throughs (config 4) from the line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5),
                     help="3: 2048^3 full decode + 100k batched requests (default); 2: 512^3 demo scale; "
@@ -434,6 +439,24 @@ def bench_config3(args):
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_config3(args, data, ds, nb, L)
+    if not args.no_config4:
+        # the metric's "chosen vs plain GPU form": learned prefetch against the
+        # plain on-demand form on the 1024^3 fly-throughs (config 4, full size)
+        vol.close()
+        ctx.close()
+        c4 = run_config4(args)
+        out["chosen_vs_plain_1080p"] = {
+            "workload": c4["config"]["workload"],
+            "fps_plain_on_demand": c4["fps_plain_on_demand"],
+            "fps_visibility_driven": c4["fps_visibility_driven"],
+            "fps_learned_prefetch": c4["fps_learned_prefetch"],
+            "speedup_learned_vs_plain": c4["speedup_vs_plain"],
+            "speedup_visibility_driven_vs_plain": c4["speedup_visibility_driven_vs_plain"],
+            "device_ms_per_frame": c4["device_ms_per_frame"],
+            "on_demand_decodes": c4["on_demand_decodes"],
+            "miss_rate": c4["miss_rate"], "wasted_frac": c4["wasted_frac"],
+            "label_mismatch_frames": c4["label_mismatch_frames"],
+        }
     print(json.dumps(out))
 
 
@@ -508,6 +531,10 @@ def cpu_baseline_config3(args, data, ds, nb, L):
 
 
 def bench_config4(args):
+    print(json.dumps(run_config4(args)))
+
+
+def run_config4(args):
     """1024^3 u32 volume (seed 3, the 512^3 recipe at the same seed density),
     32^3 bricks (32,768), direct-mapped 4 GiB brick cache. Camera paths are
     Catmull-Rom splines through 8 seeded random control points inside the
@@ -592,7 +619,10 @@ def bench_config4(args):
         "train": {"samples": n_samples, "epochs": args.epochs, "seconds": train_s, "loss_first": float(losses[0]),
                   "loss_last": float(losses[-1]), "visit_log_seconds": collect_s},
     }
-    print(json.dumps(out))
+    mlp.close()
+    vol.close()
+    ctx.close()
+    return out
 
 def bench_config5(args):
     """Serving: 64 viewer sessions over four volumes of the 512^3 recipe at

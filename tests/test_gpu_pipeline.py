@@ -3,6 +3,8 @@ streams on the context stream and the rest on a second stream, overlapped.
 Bit-exact against decode_rans + decode_all (and the oracle): pyramids and
 single levels, both rANS mappings, both label widths, corrupt streams, and a
 512^3 volume of 64^3 bricks whose stream lengths really split."""
+import math
+
 import numpy as np
 import pytest
 
@@ -151,3 +153,42 @@ def test_decode_volume_direct_pool(ctx, manifest, name):
         ds = O.Dataset(data, b)
         for i in (0, ds.num_bricks - 1):
             assert np.array_equal(out[0][i].astype(np.uint64), ds.decode(i, ds.L - lod_drop)[0])
+
+
+def test_decode_volume_repeated_calls_and_residency_changes(ctx):
+    """Repeated calls, and calls after a residency change in between --
+    requests at another LOD, another LOD for the whole volume, a
+    direct-mapped pool, a device-built frame that re-decodes bricks at their
+    frame LOD: every call leaves every brick's finest labels equal to the
+    oracle's and every brick resident at that LOD."""
+    print("[gpu] decode_volume repeated calls", flush=True)
+    data = golden_bytes("cfg1_b32")
+    ds = O.Dataset(data, 32)
+    vol = ctx.load(data, 32)
+    try:
+        L, nb = vol.info["levels"], vol.info["num_bricks"]
+        per = 1 << (3 * L)
+        want = np.stack([ds.decode(i, L)[0] for i in range(nb)])
+
+        def check(stride):
+            assert np.all(vol.decode_volume(L, False) == 0)
+            got = np.stack([vol.read_cache(i * stride, per) for i in range(nb)]).astype(np.uint64)
+            assert np.array_equal(got, want)
+            bricks, lods = vol.cache_resident()
+            assert len(bricks) == nb and np.all(lods == L)
+
+        for _ in range(3):
+            check(per)
+        vol.cache_clear()  # (a full decode leaves the pool exactly full)
+        vol.request_bricks(np.arange(8, dtype=np.uint32), np.full(8, 2, np.uint8))
+        check(per)
+        assert np.all(vol.decode_volume(2, False) == 0)
+        check(per)
+        vol.cache_reserve_direct()
+        check(per)
+        check(per)
+        cam = vol.orbit_camera(0.7, 0.3, 3.0 * vol.info["grid_size"], math.pi / 3)
+        vol.render(cam, 160, 120, want=())  # device-built frame: bricks re-decoded at their frame LOD
+        check(per)
+    finally:
+        vol.close()
