@@ -61,7 +61,7 @@ using CfgSmall4 = OpsCfg<256, 4096, 4096, 0, 4096, false, 128, 1024, 4>;
 // 64^3 bricks: sixteen waves, the 32768-node level in LDS (~155 KB)
 using CfgLarge = OpsCfg<1024, 4096, 0, 32768, 32768, false, 1024, 0>;
 // head pass: one wave decodes levels 1..3 of one request (~6.5 KB)
-using CfgHead = OpsCfg<64, 512, 0, 0, 512, true, 128, 0>;
+using CfgHead = OpsCfg<64, 512, 512, 0, 512, true, 128, 0>;
 
 constexpr uint32_t SCR_WORDS = 256;
 constexpr uint32_t SCR_WAVESUM = 0;   // NW words: active-node counts per wave
@@ -90,6 +90,10 @@ struct Layout {
   static constexpr size_t spal = (ring + 192 + 15) / 16 * 16;
   static constexpr size_t bytes = spal + C::PBYTES;
   static constexpr uint32_t PCAP = C::PBYTES / sizeof(LT);
+  // kstop (XCAP / 32 words) ends the active list; a level parsed in compact
+  // form has at most XCAP / 8 active parents, which must stay clear of it
+  static_assert(C::XCAP / 8 * (C::PACKED ? 4 : 2) + C::XCAP / 8 <= C::ACT * (C::PACKED ? 4 : 2),
+                "compact-level stop bits overlap the active list");
 };
 
 template <typename LT>
@@ -100,6 +104,8 @@ struct Lds {
   uint32_t* act32;  // active parents of the level being expanded (Morton order): node | value << 16
   uint16_t* act16;  // same, node only
   uint16_t* wpre;   // active parents before each stop word (rank lookup)
+  uint32_t* kstop;  // stop bits in op order of a non-final level parsed in compact form (the
+                    // active list's tail, unused while that level's parents are at most XCAP / 8)
   uint32_t* st0;
   uint32_t* st1;
   uint32_t* scr;
@@ -118,6 +124,7 @@ __device__ __forceinline__ Lds<LT> make_lds(uint8_t* smem) {
   S.act32 = reinterpret_cast<uint32_t*>(smem + Y::act);
   S.act16 = reinterpret_cast<uint16_t*>(smem + Y::act);
   S.wpre = reinterpret_cast<uint16_t*>(smem + Y::wpre);
+  S.kstop = reinterpret_cast<uint32_t*>(smem + Y::wpre) - C::XCAP / 32;
   S.st0 = reinterpret_cast<uint32_t*>(smem + Y::st0);
   S.st1 = reinterpret_cast<uint32_t*>(smem + Y::st1);
   S.scr = reinterpret_cast<uint32_t*>(smem + Y::scr);
@@ -127,7 +134,12 @@ __device__ __forceinline__ Lds<LT> make_lds(uint8_t* smem) {
   return S;
 }
 
-enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2 };
+// MODE_LEVEL: ops resolved per nibble into the next level's node array;
+// MODE_LCOMPACT: a non-final level parsed like MODE_COMPACT (op nibbles in op
+// order), then resolved per active parent into the next level's node array;
+// MODE_COMPACT / MODE_GLOBAL: the final level, op results per op in LDS or
+// per node in the cache region.
+enum : uint32_t { MODE_LEVEL = 0, MODE_COMPACT = 1, MODE_GLOBAL = 2, MODE_LCOMPACT = 3 };
 
 // The same configuration run by one wave (the first of the workgroup): its
 // LDS layout is the workgroup's, its thread count 64 and its syncs wave-level.
@@ -343,8 +355,9 @@ struct LevelIO {
   const uint16_t* vcur;
   uint16_t* vnext;   // MODE_LEVEL: next level values
   uint32_t* snext;   // MODE_LEVEL: next level stop bits
-  uint16_t* opv;     // MODE_COMPACT: per-op results
+  uint16_t* opv;     // MODE_COMPACT / MODE_LCOMPACT: per-op results
   LT* olev;          // MODE_GLOBAL: per-node results in the cache region
+  uint32_t* kstop;   // MODE_LCOMPACT: stop bits in op order (cleared before the parse)
 };
 
 // Parse and resolve the ops of one level (P1). Returns the number of ops
@@ -459,10 +472,12 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
         pv1 = io.vcur[m1];
       }
       bool e_pal = false, e_ovr = false, e_op = false;
-      if constexpr (MODE == MODE_COMPACT) {
+      if constexpr (MODE == MODE_COMPACT || MODE == MODE_LCOMPACT) {
         // the op nibble as it is, tagged with bit 15 (palette results, all
-        // below 0x8000, overwrite their slots below); resolve_compact turns
-        // it into a value or a node reference once the level is parsed
+        // below 0x8000, overwrite their slots below); resolve_compact /
+        // resolve_level turn it into a value or a node reference once the
+        // level is parsed
+        uint32_t sbits = 0;  // MODE_LCOMPACT: stop bits of the lane's ops, in op order
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
           const uint32_t v = (word >> (4 * j)) & 0xFu;
@@ -470,7 +485,16 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
           const bool live = isop && k < K;
           e_op = e_op || (live && (v >> 1) > FCS_OP_PD);
           *(live ? &io.opv[k] : dum16) = (uint16_t)(FCS_REF16 | v);
+          if constexpr (MODE == MODE_LCOMPACT) sbits |= (live && (v & 1u)) ? (1u << (k - k0)) : 0u;
           k += isop ? 1u : 0u;
+        }
+        if constexpr (MODE == MODE_LCOMPACT) {
+          // at most 8 consecutive op indices from k0: one or two words
+          if (sbits != 0u) {
+            const uint64_t sh = (uint64_t)sbits << (k0 & 31u);
+            atomicOr(&io.kstop[k0 >> 5], (uint32_t)sh);
+            if ((uint32_t)(sh >> 32)) atomicOr(&io.kstop[(k0 >> 5) + 1], (uint32_t)(sh >> 32));
+          }
         }
       } else {
 #pragma unroll
@@ -539,7 +563,7 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
           const uint32_t n = (m << 3) | (kj & 7u);
           if constexpr (MODE == MODE_LEVEL) {
             io.vnext[n] = (uint16_t)pval;
-          } else if constexpr (MODE == MODE_COMPACT) {
+          } else if constexpr (MODE == MODE_COMPACT || MODE == MODE_LCOMPACT) {
             io.opv[kj] = (uint16_t)pval;
           } else {
             io.olev[n] = (LT)pval;
@@ -573,17 +597,70 @@ __device__ __forceinline__ uint32_t parse_level(const Lds<LT>& S, const LevelIO<
 }
 
 
-// Compact op results of the final level, one active parent (8 children, 8
-// consecutive ops) per step: the op nibbles left by parse_level (bit 15 set)
-// become values or node references, palette results stay. Rx/Ry/Rz name the
-// neighbour across the face of the child's own 2x2x2 block: children with the
-// axis bit clear reference the matching child of the parent's previous
-// neighbour along that axis, children with the bit set take the value of the
+// One active parent's 8 op results (8 consecutive ops = its children in
+// order, packed two per word in w): the op nibbles left by parse_level (bit 15
+// set) become values or node references, palette results stay. Rx/Ry/Rz name
+// the neighbour across the face of the child's own 2x2x2 block: children with
+// the axis bit clear reference the matching child of the parent's previous
+// neighbour along that axis, children with it set take the value of the
 // parent's next neighbour (that level is not decoded yet). Outside the brick
 // the reference reads the parent's output slot, which child 0 has already
 // overwritten when c > 0. The neighbours are worked out once per parent; a
 // reference into a stopped parent takes that parent's value at once (its
 // children all copy it), and one to child 0 takes child 0's result.
+__device__ __forceinline__ void resolve_group(uint32_t (&w)[4], uint32_t m, uint32_t pv, const uint16_t* vcur,
+                                              const uint32_t* scur, uint32_t pmask) {
+  uint32_t prev[3], next[3];
+  bool first[3], last[3];
+#pragma unroll
+  for (uint32_t ax = 0; ax < 3; ++ax) {
+    // (the neighbours wrap inside the level at the faces, where they are unused)
+    const uint32_t mA = (0x09249249u << ax) & pmask;
+    const uint32_t xa = m & mA, rest = m & ~mA;
+    const uint32_t mp = ((xa - 1u) & mA) | rest;
+    first[ax] = xa == 0u;
+    last[ax] = xa == mA;
+    const bool pstop = (scur[mp >> 5] >> (mp & 31)) & 1u;
+    const uint32_t pval = vcur[mp];
+    prev[ax] = pstop ? pval : (FCS_REF16 | (8u * mp));
+    next[ax] = vcur[(((xa | ~mA) + 1u) & mA) | rest];
+  }
+  uint32_t v0 = pv;  // child 0's result
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const uint32_t sh = 16 * (c & 1);
+    const uint32_t x = (w[c >> 1] >> sh) & 0xFFFFu;
+    const uint32_t oval = (c == 0) ? pv : v0;
+    uint32_t cand[3];
+#pragma unroll
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+      const uint32_t pb = (prev[ax] & FCS_REF16) ? (prev[ax] | (c | (1u << ax))) : prev[ax];
+      cand[ax] = ((c >> ax) & 1u) ? (last[ax] ? oval : next[ax]) : (first[ax] ? oval : pb);
+    }
+    const uint32_t op = (x >> 1) & 0x7u;
+    const uint32_t lo = (op & 1u) ? cand[0] : pv;       // Rx : Rp
+    const uint32_t hi = (op & 1u) ? cand[2] : cand[1];  // Rz : Ry
+    const uint32_t r = (op & 4u) ? 0u : ((op & 2u) ? hi : lo);  // corrupt op nibbles: 0
+    const uint32_t val = (x & FCS_REF16) ? r : x;
+    if (c == 0) v0 = val;
+    w[c >> 1] = (w[c >> 1] & ~(0xFFFFu << sh)) | (val << sh);
+  }
+}
+
+template <typename LT, class C>
+__device__ __forceinline__ void act_entry(const Lds<LT>& S, const uint16_t* vcur, uint32_t gp, uint32_t& m,
+                                          uint32_t& pv) {
+  if constexpr (C::PACKED) {
+    const uint32_t e = S.act32[gp];
+    m = e & 0xFFFFu;
+    pv = e >> 16;
+  } else {
+    m = S.act16[gp];
+    pv = vcur[m];
+  }
+}
+
+// Final level: each active parent's group of 8 op results is resolved in place.
 template <typename LT, class C>
 __device__ __forceinline__ void resolve_compact(const Lds<LT>& S, uint16_t* opv, const uint16_t* vcur,
                                                 const uint32_t* scur, uint32_t A, uint32_t l) {
@@ -592,51 +669,31 @@ __device__ __forceinline__ void resolve_compact(const Lds<LT>& S, uint16_t* opv,
     const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
     if (!((q.x | q.y | q.z | q.w) & 0x80008000u)) continue;  // palette results only
     uint32_t m, pv;
-    if constexpr (C::PACKED) {
-      const uint32_t e = S.act32[gp];
-      m = e & 0xFFFFu;
-      pv = e >> 16;
-    } else {
-      m = S.act16[gp];
-      pv = vcur[m];
-    }
-    uint32_t prev[3], next[3];
-    bool first[3], last[3];
-#pragma unroll
-    for (uint32_t ax = 0; ax < 3; ++ax) {
-      // (the neighbours wrap inside the level at the faces, where they are unused)
-      const uint32_t mA = (0x09249249u << ax) & pmask;
-      const uint32_t xa = m & mA, rest = m & ~mA;
-      const uint32_t mp = ((xa - 1u) & mA) | rest;
-      first[ax] = xa == 0u;
-      last[ax] = xa == mA;
-      const bool pstop = (scur[mp >> 5] >> (mp & 31)) & 1u;
-      const uint32_t pval = vcur[mp];
-      prev[ax] = pstop ? pval : (FCS_REF16 | (8u * mp));
-      next[ax] = vcur[(((xa | ~mA) + 1u) & mA) | rest];
-    }
+    act_entry<LT, C>(S, vcur, gp, m, pv);
     uint32_t w[4] = {q.x, q.y, q.z, q.w};
-    uint32_t v0 = pv;  // child 0's result
-#pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) {
-      const uint32_t sh = 16 * (c & 1);
-      const uint32_t x = (w[c >> 1] >> sh) & 0xFFFFu;
-      const uint32_t oval = (c == 0) ? pv : v0;
-      uint32_t cand[3];
-#pragma unroll
-      for (uint32_t ax = 0; ax < 3; ++ax) {
-        const uint32_t pb = (prev[ax] & FCS_REF16) ? (prev[ax] | (c | (1u << ax))) : prev[ax];
-        cand[ax] = ((c >> ax) & 1u) ? (last[ax] ? oval : next[ax]) : (first[ax] ? oval : pb);
-      }
-      const uint32_t op = (x >> 1) & 0x7u;
-      const uint32_t lo = (op & 1u) ? cand[0] : pv;       // Rx : Rp
-      const uint32_t hi = (op & 1u) ? cand[2] : cand[1];  // Rz : Ry
-      const uint32_t r = (op & 4u) ? 0u : ((op & 2u) ? hi : lo);  // corrupt op nibbles: 0
-      const uint32_t val = (x & FCS_REF16) ? r : x;
-      if (c == 0) v0 = val;
-      w[c >> 1] = (w[c >> 1] & ~(0xFFFFu << sh)) | (val << sh);
-    }
+    resolve_group(w, m, pv, vcur, scur, pmask);
     *reinterpret_cast<uint4*>(opv + 8 * gp) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// Non-final level parsed in compact form: each active parent's 8 resolved
+// results go to its children in the next level's node array (one 16-byte
+// store), and their stop bits (op order = child order) to the children's
+// byte of the next level's stop words, which the previous level preset to 0
+// for children of active parents.
+template <typename LT, class C>
+__device__ __forceinline__ void resolve_level(const Lds<LT>& S, const uint16_t* opv, const uint16_t* vcur,
+                                              const uint32_t* scur, uint16_t* vnext, uint32_t* snext, uint32_t A,
+                                              uint32_t l) {
+  const uint32_t pmask = (1u << (3 * l)) - 1u;
+  for (uint32_t gp = threadIdx.x; gp < A; gp += C::NT) {
+    const uint4 q = *reinterpret_cast<const uint4*>(opv + 8 * gp);
+    uint32_t m, pv;
+    act_entry<LT, C>(S, vcur, gp, m, pv);
+    uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    if ((q.x | q.y | q.z | q.w) & 0x80008000u) resolve_group(w, m, pv, vcur, scur, pmask);
+    *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint8_t*>(snext)[m] = (uint8_t)(S.kstop[gp >> 2] >> (8 * (gp & 3u)));
   }
 }
 
@@ -691,11 +748,19 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   const bool last = (l + 1 == t);
   uint32_t mode = MODE_LEVEL;
   if constexpr (!ONE) mode = !last ? MODE_LEVEL : ((C::KCAP != 0 && K <= C::KCAP) ? MODE_COMPACT : MODE_GLOBAL);
+  // A level decoded by one wave whose ops fit the area between the ping-pong
+  // buffers is parsed in compact form and resolved per active parent
+  // (MODE_LCOMPACT): fewer instructions than the per-nibble resolve, and one
+  // wave has no barrier to pay for the extra phase. (A workgroup's level
+  // step measured the same either way and stays per nibble.)
+  bool lcomp = false;
+  if constexpr (ONE && C::XCAP != 0) lcomp = !last && K <= C::XCAP && Cn <= C::VCAP;
   LT* olev = c.o + (c.pyr ? fcs_pyramid_level_offset(l + 1) : 0ull);
   uint16_t* vcur = v.vcur;
   uint16_t* vnext = (Cn <= C::VCAP) ? (vcur == S.sA ? S.sB : S.sA) : S.big;
   // compact op results: the free ping-pong buffer and the area next to it
-  uint16_t* opv = (vcur == S.sA) ? S.sA + C::VCAP : S.sA;
+  // (for a non-final level, the area alone: both buffers hold levels)
+  uint16_t* opv = (lcomp || vcur == S.sA) ? S.sA + C::VCAP : S.sA;
   uint32_t* scur = v.scur;
   uint32_t* snext = v.snext;
   const uint32_t A = v.A;
@@ -707,11 +772,15 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
         const uint32_t x = vcur[m] | ((uint32_t)vcur[m] << 16);
         *reinterpret_cast<uint4*>(vnext + 8 * m) = make_uint4(x, x, x, x);
       }
+    if (lcomp)
+      for (uint32_t i = tid; i < (K + 31) / 32; i += NT) S.kstop[i] = 0u;  // ordered by the parse's first sync
   }
-  LevelIO<LT> io{c.nibw, v.ns, nsym, c.w_end, K, v.ip0, P, (1u << (3 * (l + 1))) - 1u, vcur, vnext, snext, opv, olev};
+  LevelIO<LT> io{c.nibw, v.ns, nsym, c.w_end, K, v.ip0, P, (1u << (3 * (l + 1))) - 1u, vcur, vnext, snext, opv, olev,
+                 S.kstop};
   uint32_t found;
   if constexpr (ONE) {
-    found = parse_level<MODE_LEVEL, LT, CS>(S, io, blk, v.tile, status);
+    if (lcomp) found = parse_level<MODE_LCOMPACT, LT, CS>(S, io, blk, v.tile, status);
+    else found = parse_level<MODE_LEVEL, LT, CS>(S, io, blk, v.tile, status);
   } else {
     if (mode == MODE_LEVEL) found = parse_level<MODE_LEVEL, LT, CS>(S, io, blk, v.tile, status);
     else if (mode == MODE_COMPACT) found = parse_level<MODE_COMPACT, LT, CS>(S, io, blk, v.tile, status);
@@ -725,6 +794,11 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   if constexpr (!ONE) {
     if (mode == MODE_COMPACT) {
       resolve_compact<LT, C>(S, opv, vcur, scur, A, l);
+      group_sync<CS>();
+    }
+  } else {
+    if (lcomp) {
+      resolve_level<LT, CS>(S, opv, vcur, scur, vnext, snext, A, l);
       group_sync<CS>();
     }
   }
