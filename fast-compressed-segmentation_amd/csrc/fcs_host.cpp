@@ -99,7 +99,6 @@ struct fcs_volume {
   uint32_t* d_rtab = nullptr;  // x -> (symbol << 25) | (freq << 12) | (x - cum)
   uint32_t* d_rlist = nullptr; // bricks by nibble count, longest first (the rANS launch order)
   std::vector<uint32_t> rlist; // host copy
-  uint32_t n_long = 0;         // leading entries of rlist at least half as long as the longest
   uint32_t* d_counter = nullptr;
   bool rans_ready = false;
   int rans_tuned = 0;  // mapping picked by the autotuner for this volume (0 = not yet)
@@ -894,8 +893,6 @@ static int load_csdb_body(fcs_ctx* c, const uint8_t* bytes, size_t n, uint32_t b
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return v->hb[x].nsym > v->hb[y].nsym; });
     if (int rc2 = dev_upload(c, &v->d_rlist, order.data(), order.size() * 4)) { fcs_volume_free(v); return rc2; }
-    const uint64_t longest = v->hb[order[0]].nsym;
-    while (v->n_long < v->nb && 2ull * v->hb[order[v->n_long]].nsym >= longest) ++v->n_long;
     v->rlist = std::move(order);
   }
   v->pal_host = std::move(hp);
@@ -1308,13 +1305,12 @@ int fcs_decode_all(fcs_volume* v, uint32_t lod, int pyramid) {
   return fcs_guard(v ? v->ctx : nullptr, [&] { return decode_all_body(v, lod, pyramid); });
 }
 
-// Both stages for every brick, overlapped. The streams at least half as long
-// as the longest go through rANS and the op decoder on the context stream; the
-// rest on the aux stream, whose rANS and op decode run while the long streams
-// are still in their rANS stage (a stream's rANS time grows with its length;
-// the op decode of a short brick does not wait for the long ones). Bricks
-// above 32^3 decode with a workgroup that takes a whole CU's LDS, which cannot
-// start beside the rANS launch: there both stages run on the context stream.
+// Both stages for every brick in one call, on the context stream with no host
+// synchronisation between them: the rANS launch over every stream (longest
+// first), then the op decoder over every brick in the same order. Running the
+// short streams' two stages on a second stream beside the long streams' rANS
+// stage measured slower than this order once the rANS stage had become short,
+// and one call in three much slower: the two stages compete for the same LDS.
 static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t* out_status) {
   if (!v || !v->ctx) return FCS_E_INVALID;
   fcs_ctx* c = v->ctx;
@@ -1331,34 +1327,19 @@ static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t
   }
   std::vector<DecodeReq> reqs;
   if (int rc = full_requests(v, lod, pyramid, reqs)) return rc;
-  const uint32_t nl = v->b <= 32 ? v->n_long : v->nb;
-  std::vector<DecodeReq> rl, rs;
-  rl.reserve(nl);
-  rs.reserve(v->nb - nl);
-  for (uint32_t k = 0; k < v->nb; ++k) (k < nl ? rl : rs).push_back(reqs[v->rlist[k]]);
-  // requests go up before any launch: copies from pageable memory would
-  // otherwise hold the host until the stream before them has drained
+  std::vector<DecodeReq> rl;
+  rl.reserve(v->nb);
+  for (uint32_t k = 0; k < v->nb; ++k) rl.push_back(reqs[v->rlist[k]]);
+  // requests go up before any launch
   if (int rc = stage_decode(v, v->lane_main, rl, c->stream)) return rc;
-  if (int rc = stage_decode(v, v->lane_aux, rs, c->aux)) return rc;
-  const RansArgs a = rans_args(v);
-  RansArgs al = a, as = a;
-  al.nlist = nl;
-  as.list = v->d_rlist + nl;
-  as.nlist = v->nb - nl;
   HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  HIP_TRY(c, hipEventRecord(c->main_mark, c->stream));
-  HIP_TRY(c, hipStreamWaitEvent(c->aux, c->main_mark, 0));
-  HIP_TRY(c, fcs_launch_rans(as, mode, c->aux));
-  if (int rc = issue_decode(v, v->lane_aux, c->aux, false, 0, false)) return rc;
-  HIP_TRY(c, hipEventRecord(c->aux_done, c->aux));
-  HIP_TRY(c, fcs_launch_rans(al, mode, c->stream));
+  HIP_TRY(c, fcs_launch_rans(rans_args(v), mode, c->stream));
   if (int rc = issue_decode(v, v->lane_main, c->stream, false, 0, false)) return rc;
-  HIP_TRY(c, hipStreamWaitEvent(c->stream, c->aux_done, 0));
   HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  for (DecodeLane* ln : {&v->lane_main, &v->lane_aux})
-    if (!ln->reqs.empty())
-      HIP_TRY(c, hipMemcpyAsync(ln->status_sorted.data(), ln->d_status, ln->reqs.size() * sizeof(uint32_t),
-                                hipMemcpyDeviceToHost, c->stream));
+  DecodeLane& ln = v->lane_main;
+  if (!ln.reqs.empty())
+    HIP_TRY(c, hipMemcpyAsync(ln.status_sorted.data(), ln.d_status, ln.reqs.size() * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
   bool rbad = false;
   if (int rc = settle_rans(v, nullptr, &rbad)) return rc;  // synchronises the context stream
   float ms = 0;
@@ -1366,14 +1347,12 @@ static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t
   c->timing.pipeline_ms = ms;
   c->timing.rans_mapping = (uint32_t)mode;
   account_decode(v, reqs);
-  std::vector<uint32_t> sl, ss;
-  finish_decode(v->lane_main, sl);
-  finish_decode(v->lane_aux, ss);
-  v->lane_aux.reqs.clear();
+  std::vector<uint32_t> sl;
+  finish_decode(ln, sl);
   uint32_t nbad = 0, any = 0, first = ~0u;
   for (uint32_t k = 0; k < v->nb; ++k) {
     const uint32_t b = v->rlist[k];
-    const uint32_t st = v->rans_status[b] | (k < nl ? sl[k] : ss[k - nl]);
+    const uint32_t st = v->rans_status[b] | sl[k];
     if (out_status) out_status[b] = st;
     if (st) {
       ++nbad;

@@ -344,11 +344,13 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
 // requested, and finished nibble words are stored. Waits for global memory
 // (which complete in issue order, per wave) therefore only ever cover
 // operations issued a whole interval earlier; a lane refilling its buffer at
-// its own moment waits for LDS only. A valid stream consumes at most 2 bytes
-// per symbol (the state never falls below 2^11 before renormalising), so a
-// ring holding two intervals never runs dry; a corrupt one that does
-// continues with direct byte loads, so every stream decodes exactly as the
-// sequential rule reads it.
+// its own moment waits for LDS only. While the state is at least 2^23 before
+// every symbol, a symbol leaves it at least 2^11 and renormalising takes at
+// most 2 bytes, so a ring holding two intervals never runs dry. A stream that
+// breaks this (an initial state below 2^23, or one that ends before its
+// symbols do: corrupt input only) marks its lane, which then decodes the
+// brick again by the sequential rule, byte by byte, so every stream decodes
+// exactly as that rule reads it.
 constexpr uint32_t LANE_NT = 256;
 constexpr uint32_t LR_BLOCKS = 4;             // 64 bytes per lane
 constexpr uint32_t LR_WORDS = LR_BLOCKS * 4;
@@ -358,17 +360,18 @@ constexpr uint32_t LR_MASK = (LR_WORDS - 1u) * LR_ROW;
 
 struct LaneStream {
   const uint4* base;  // the brick's stream as 16-byte blocks
-  const uint8_t* bytes;
   char* ring;         // LDS, LR_WORDS rows of LANE_NT words, this lane's column
   int32_t q;          // next block to request (decreasing; below 0 reads as zeros)
   // ring words read / written so far, times LR_ROW (so that the low bits
-  // masked by LR_MASK are the row's byte offset); rd never passes wr, and
-  // they differ by at most LR_WORDS words, so comparisons survive wrapping
+  // masked by LR_MASK are the row's byte offset); on a lane not marked bad,
+  // rd never passes wr and they differ by at most LR_WORDS words, so
+  // comparisons survive wrapping (a bad lane's values are discarded)
   uint32_t rdx, wrx;
   uint64_t bb;        // upcoming stream bytes, the next one in bits 63..56
   uint32_t nbits;     // valid bits in bb (a multiple of 8)
   uint32_t pre;       // ring word rd, read at the end of the previous symbol
-  bool pre_ok, staged, direct;
+  bool staged;
+  bool bad;           // the fast rule left the sequential one: redo the brick
   uint4 s0, s1;       // blocks requested at the last checkpoint
 
   __device__ __forceinline__ uint4 blk(int32_t k) const {
@@ -388,25 +391,17 @@ struct LaneStream {
   }
   // issued once per symbol, by every lane: the word a refill would take next
   // (read again unchanged when nothing was taken); used one symbol later
-  __device__ __forceinline__ void prefetch() {
-    pre = at(rdx);
-    pre_ok = rdx != wrx;
-  }
-  // fewer than 2 bytes left: append the prefetched word below them. A ring
-  // run dry (a corrupt stream only) switches the lane to direct byte loads.
+  __device__ __forceinline__ void prefetch() { pre = at(rdx); }
+  // fewer than 2 bytes left after the initial state: append the prefetched
+  // word below them
   __device__ __forceinline__ void refill() {
-    if (pre_ok) {
-      bb |= (uint64_t)pre << (32u - nbits);
-      nbits += 32u;
-      rdx += LR_ROW;
-      pre_ok = false;
-    } else {
-      direct = true;
-    }
+    bad = bad || rdx == wrx;
+    bb |= (uint64_t)pre << (32u - nbits);
+    nbits += 32u;
+    rdx += LR_ROW;
   }
   __device__ __forceinline__ void init(const uint8_t* s, uint32_t E) {
     base = reinterpret_cast<const uint4*>(s);
-    bytes = s;
     const int32_t q0 = (int32_t)((E - 1) >> 4);
     wrx = 0;
     for (uint32_t k = 0; k < LR_BLOCKS; ++k) put(blk(q0 - (int32_t)k));
@@ -417,7 +412,7 @@ struct LaneStream {
     s1 = blk(q - 1);
     q -= 2;
     staged = true;
-    direct = false;
+    bad = false;
     const uint32_t j = ((E - 1) >> 2) & 3u;     // word of byte E-1 in its block
     const uint32_t skip = 3u - ((E - 1) & 3u);  // bytes of that word beyond E
     rdx = (3u - j) * LR_ROW;
@@ -444,36 +439,32 @@ struct LaneStream {
       staged = true;
     }
   }
-  // renormalise after a symbol: while state < 2^23 and bytes are left, shift
-  // the next byte in. From a state of at least 2^7 that is n = [s < 2^23] +
-  // [s < 2^15] <= 2 bytes, taken from bb at once; a smaller state (a corrupt
-  // stream) continues with direct loads, byte by byte, as the sequential rule.
-  // Straight-line code but for that rare loop: a lane in direct mode takes
-  // n = 0 and no refill.
-  __device__ __forceinline__ void renorm(uint32_t& state, uint32_t& pos) {
-    direct = direct || state < 128u;
-    const uint32_t nn = min((state < FCS_RANS_L ? 1u : 0u) + (state < (1u << 15) ? 1u : 0u), pos);
-    const uint32_t n = direct ? 0u : nn;
-    const uint32_t sh = 8u * n;
-    // (state : top word of bb) >> (32 - sh): the state shifted up by sh bits
-    // with the next n bytes below
-    state = (uint32_t)((((uint64_t)state << 32) | (bb >> 32)) >> (32u - sh));
-    bb <<= sh;
-    nbits -= sh;
+  // renormalise after a symbol: while state < 2^23, shift the next byte in.
+  // A state in [2^11, 2^31) takes (clz(state) - 1) rounded down to a multiple
+  // of 8 bits: 0, 1 or 2 bytes, shifted in from bb at once (bb holds at least
+  // 16 bits here). A state outside that range or fewer bytes left than taken
+  // happen on corrupt input only and mark the lane for the sequential redo.
+  // A lane not marked keeps its state at least 2^23 before each symbol, so a
+  // symbol leaves it in [2^11, 2^31).
+  __device__ __forceinline__ void shift(uint32_t& state, uint32_t& pos) {
+    const uint32_t sh = ((uint32_t)__clz((int)state) - 1u) & ~7u;
+    const uint32_t n = sh >> 3;
+    bad = bad || n > pos;  // also a state of 2^31 or more (sh wraps)
     pos -= n;
-    const bool low = !direct && nbits < 16u;
-    const bool ok = low && pre_ok;
-    const uint32_t pw = ok ? pre : 0u;
-    bb |= (uint64_t)pw << ((32u - nbits) & 63u);
-    nbits += ok ? 32u : 0u;
-    rdx += ok ? LR_ROW : 0u;
-    direct = direct || (low && !ok);  // a ring run dry: a corrupt stream
-    if (direct) {
-      while (state < FCS_RANS_L && pos > 0) {
-        state = (state << 8) | bytes[pos - 1];
-        --pos;
-      }
-    }
+    state = (uint32_t)(((((uint64_t)state) << 32) | (bb >> 32)) >> ((32u - sh) & 63u));
+    bb <<= sh & 63u;
+    nbits -= sh;
+  }
+  // fewer than T bits left (T <= 32): append the word prefetched from the
+  // ring below them. With at most 2 bytes per symbol and a refill of one
+  // word, a lane not marked reads at most 8 ring words per 16 symbols, which
+  // the checkpoints keep staged ahead of it: its ring never runs dry.
+  template <uint32_t T>
+  __device__ __forceinline__ void top_up() {
+    const bool low = nbits < T;
+    bb |= (uint64_t)(low ? pre : 0u) << ((32u - nbits) & 63u);
+    nbits += low ? 32u : 0u;
+    rdx += low ? LR_ROW : 0u;
   }
 };
 
@@ -483,6 +474,30 @@ __device__ __forceinline__ uint32_t lane_step(uint32_t& state, const uint32_t* t
   const uint32_t e = table[state & (FCS_RANS_M - 1)];
   state = (e >> 16) * (state >> FCS_RANS_LOGM) + ((e >> 4) & 0xFFFu);
   return e;
+}
+
+// A corrupt stream's brick decoded again by the sequential rule (a byte at a
+// time while the state is below 2^23 and bytes are left), over the nibble
+// words the fast pass wrote; returns the stream's status bits.
+__device__ __noinline__ uint32_t lane_redo(const uint8_t* in, uint32_t E, uint32_t nsym, uint32_t* nibw,
+                                           const uint32_t* table) {
+  uint32_t pos = E - 4;
+  uint32_t state = ((uint32_t)in[E - 1] << 24) | ((uint32_t)in[E - 2] << 16) | ((uint32_t)in[E - 3] << 8) | in[E - 4];
+  auto step = [&]() -> uint32_t {
+    const uint32_t e = lane_step(state, table);
+    while (state < FCS_RANS_L && pos > 0) state = (state << 8) | in[--pos];
+    return e & 15u;
+  };
+  for (uint32_t wd = 0; 8 * wd < nsym; ++wd) {
+    const uint32_t cnt = min(8u, nsym - 8 * wd);
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < cnt; ++k) word = (word >> 4) | (step() << 28);
+    nibw[wd] = word >> (32u - 4u * cnt);
+  }
+  uint32_t status = 0;
+  if ((nsym & 1u) && step() != 0) status |= FCS_ST_RANS_STATE;
+  if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  return status;
 }
 
 __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
@@ -512,12 +527,29 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   uint32_t state = (uint32_t)(rd.bb >> 32);
   rd.bb <<= 32;
   rd.nbits -= 32u;
-  if (rd.nbits < 16u) rd.refill();
+  // at least 32 bits for the first pair of symbols
+  if (rd.nbits < 32u) rd.refill();
+  rd.prefetch();
+  rd.bad = rd.bad || state < FCS_RANS_L;
+  // one symbol; at least 16 bits left for the next
   auto step = [&]() -> uint32_t {
     const uint32_t e = lane_step(state, table);
-    rd.renorm(state, pos);
+    rd.shift(state, pos);
+    rd.top_up<16u>();
     rd.prefetch();
     return e;
+  };
+  // two symbols (at most 32 bits) from a buffer of at least 32; one refill
+  // to at least 32 bits for the next pair
+  auto step2 = [&](uint32_t& word) {
+    const uint32_t e0 = lane_step(state, table);
+    rd.shift(state, pos);
+    const uint32_t e1 = lane_step(state, table);
+    rd.shift(state, pos);
+    rd.top_up<32u>();
+    rd.prefetch();
+    word = (word >> 4) | (e0 << 28);
+    word = (word >> 4) | (e1 << 28);
   };
   // 8 symbols -> one nibble word (each symbol shifted in at the top, so the
   // first ends in bits 0..3); 16 words -> one 64-byte line, stored whole. The
@@ -542,7 +574,7 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     }
     uint32_t word = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) word = (word >> 4) | (step() << 28);
+    for (uint32_t k = 0; k < 4; ++k) step2(word);
     w[wd & 15u] = word;
   }
   const uint32_t r = nsym & 7u;
@@ -561,10 +593,11 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   uint32_t status = 0;
   if (nsym & 1u) {
     const uint32_t e = lane_step(state, table);
-    rd.renorm(state, pos);
+    rd.shift(state, pos);
     if ((e & 15u) != 0) status |= FCS_ST_RANS_STATE;
   }
   if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  if (rd.bad) status = lane_redo(a.stream + B.str_off, E, nsym, nibw, table);
   a.res[b] = RansResult{status, 0u};
 }
 

@@ -205,9 +205,10 @@ class Volume:
         check(lib.fcs_decode_all(self.h, int(lod), 1 if pyramid else 0), self.ctx.h)
 
     def decode_volume(self, lod=255, pyramid=True, raise_on_error=True):
-        """decode_rans() then decode_all(lod, pyramid) in one call, the two stages
-        overlapped (fcs_decode_volume). Returns per-brick status words of both
-        stages; the context timing's pipeline_ms covers the whole call."""
+        """decode_rans() then decode_all(lod, pyramid) in one call, both launches
+        back to back on the device (fcs_decode_volume). Returns per-brick status
+        words of both stages; the context timing's pipeline_ms covers the whole
+        call."""
         st = np.zeros(self.info["num_bricks"], np.uint32)
         rc = lib.fcs_decode_volume(self.h, int(lod), 1 if pyramid else 0, _ptr(st, c_u32p))
         if rc < 0 and (raise_on_error or rc != -6):

@@ -303,8 +303,7 @@ static napi_value js_decode_all(napi_env env, napi_callback_info info) {
 }
 
 /* decodeVolume(volume, lod = 255, pyramid = true) -> Uint32Array status per
- * brick: decodeAllRansBricks then decodeAll, the two stages overlapped
- * (fcs_decode_volume) */
+ * brick: decodeAllRansBricks then decodeAll in one call (fcs_decode_volume) */
 static napi_value js_decode_volume(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;

@@ -12,7 +12,7 @@
  *                            over the decodeBrick kernel (web/shaders/decode.wgsl:344-508)
  *   fcs_decode_all        <- initializeCache(forceAll) (web/scripts/renderLoop.js:134-181)
  *   fcs_decode_volume     <- decodeAllRansBricks followed by initializeCache(forceAll) (app.js:75-82 then
- *                            renderLoop.js:134-181), the two stages overlapped
+ *                            renderLoop.js:134-181), both stages in one call
  *   fcs_read_brick        <- decodeBrick<b> (encoder/CSVEncoder/src/brickDecoding.hpp:174-242)
  *   fcs_render            <- frame(): LOD update + decode + rayTrace pass
  *                            (web/scripts/renderLoop.js:212-359, web/shaders/rayTrace.wgsl:565-906)
@@ -142,7 +142,7 @@ typedef struct {
   float tune_ops_4wg_ms;  /* autotuner: the same batch, 4 workgroups/CU */
   float tune_win_ms;      /* autotuner: whole-volume rANS time, one wave per brick, scalar byte window (table) */
   float tune_win_ballot_ms; /* autotuner: same, ballot over a per-lane CDF */
-  float pipeline_ms;      /* last fcs_decode_volume: both stages, overlapped */
+  float pipeline_ms;      /* last fcs_decode_volume: both stages */
   /* last fcs_render / fcs_render_visible march with FCS_OPT_MARCH_STATS set
    * (host-built frames): what its rays read -- label samples, non-empty
    * bricks entered (each reads its 48-byte records), empty bricks tested */
@@ -230,12 +230,10 @@ int fcs_request_bricks_static(fcs_volume* vol, const uint32_t* brick, const uint
 int fcs_decode_all(fcs_volume* vol, uint32_t lod, int pyramid);
 
 /* Every brick's rANS stage and then its decode at LOD `lod`, as fcs_decode_rans
- * followed by fcs_decode_all, with the stages overlapped: the bricks whose
- * streams are at least half as long as the longest run on the context stream,
- * the others on a second stream, where their op decode proceeds while the long
- * streams are still in the rANS stage. Same labels and statuses as the two
- * calls. out_status (nullable, num_bricks entries): per brick, the FCS_ST_*
- * bits of both stages. fcs_get_timing: pipeline_ms. */
+ * followed by fcs_decode_all, in one call: both launches go onto the context
+ * stream back to back, with no host synchronisation between them. Same labels
+ * and statuses as the two calls. out_status (nullable, num_bricks entries): per
+ * brick, the FCS_ST_* bits of both stages. fcs_get_timing: pipeline_ms. */
 int fcs_decode_volume(fcs_volume* vol, uint32_t lod, int pyramid, uint32_t* out_status);
 
 /* Copy the labels of `brick` at `lod` to the host (8^lod labels, Morton order),

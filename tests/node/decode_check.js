@@ -28,7 +28,7 @@ for (let b = 0; b < h.numBricks; b++) {
   for (let t = 0; t <= h.levels; t++) parts.push(Buffer.from(fcs.readBrick(ds, b, t).buffer));
 }
 fs.writeFileSync(path.join(outDir, 'bricks.bin'), Buffer.concat(parts));
-// both stages again in one call (overlapped), then every brick once more
+// both stages again in one call, then every brick once more
 const vst = fcs.decodeVolume(ds);
 fs.writeFileSync(path.join(outDir, 'volume_status.bin'), Buffer.from(vst.buffer));
 const parts2 = [];

@@ -3,7 +3,7 @@
 (262,144 bricks), encoded by the CPU producer.
 
 * fcs_decode_volume (the rANS stage and the finest-LOD decode of every brick,
-  the two stages overlapped on two streams) is compared with the generator's
+  both in one call) is compared with the generator's
   source bricks, every label of every brick (closed form: the finest LOD of a
   decoded brick is the source volume; decode.wgsl:344-508 and
   brickDecoding.hpp:174-242 reproduce the encoded brick exactly).

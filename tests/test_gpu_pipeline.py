@@ -1,8 +1,7 @@
-"""fcs_decode_volume: the rANS stage and the full decode in one call, the long
-streams on the context stream and the rest on a second stream, overlapped.
-Bit-exact against decode_rans + decode_all (and the oracle): pyramids and
-single levels, both rANS mappings, both label widths, corrupt streams, and a
-512^3 volume of 64^3 bricks whose stream lengths really split."""
+"""fcs_decode_volume: the rANS stage and the full decode in one call, both
+launches back to back on the context stream. Bit-exact against decode_rans +
+decode_all (and the oracle): pyramids and single levels, both rANS mappings,
+both label widths, corrupt streams, and a 512^3 volume of 64^3 bricks."""
 import math
 
 import numpy as np
