@@ -276,11 +276,14 @@ int stage_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& re
   // Bin the requests by estimated cost, heaviest first (stable within a bin):
   // the persistent decoder deals the front half round-robin and claims the
   // rest, so light requests fill the end. Cost ~ nibbles up to the LOD plus
-  // labels written.
-  constexpr int kBins = 48;
+  // labels written; bins are eighths of an octave, so requests of about the
+  // same cost are claimed together (octave bins alone left each octave in
+  // input order: the 2048^3 full decode's op kernel ran 9.5 ms in brick order
+  // against 9.15 ms sorted by stream length).
+  constexpr int kOct = 48, kSub = 8, kBins = kOct * kSub;
   std::vector<uint32_t> bin(n);
   ln.perm.resize(n);
-  size_t count[kBins + 1] = {0};
+  std::vector<size_t> count(kBins + 1, 0);
   for (size_t i = 0; i < n; ++i) {
     const DecodeReq& q = reqs[i];
     const uint32_t t = q.lod & 0xFF;
@@ -288,7 +291,9 @@ int stage_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& re
     const uint32_t drop = t < v->L ? 3 * (v->L - t) : 0;
     const uint64_t cost = (uint64_t)(nsym >> (drop < 32 ? drop : 31)) + ((1ull << (3 * (t < 20 ? t : 20))) >> 3) + 1;
     const int lg = 63 - __builtin_clzll(cost);
-    bin[i] = (uint32_t)(kBins - 1 - (lg < kBins ? lg : kBins - 1));
+    const uint32_t sub = (uint32_t)((lg >= 3 ? cost >> (lg - 3) : cost << (3 - lg)) & 7u);
+    const uint32_t key = (uint32_t)(lg < kOct ? lg : kOct - 1) * kSub + sub;  // grows with cost
+    bin[i] = (uint32_t)(kBins - 1) - key;
     ++count[bin[i] + 1];
   }
   for (int b = 0; b < kBins; ++b) count[b + 1] += count[b];

@@ -338,19 +338,21 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
 // Each lane decodes one brick. Its stream is read backwards through a ring of
 // LR_BLOCKS 16-byte blocks in LDS (kept in consumption order: ring row d % 16
 // holds the d-th word read) and a 64-bit bit buffer in registers, topped up
-// from a ring word read one symbol ahead of use. Global memory is touched only
-// every LR_CHECK symbols, an iteration all lanes of the wave share: the blocks
-// loaded at the previous checkpoint go into the ring, the next ones are
-// requested, and finished nibble words are stored. Waits for global memory
-// (which complete in issue order, per wave) therefore only ever cover
-// operations issued a whole interval earlier; a lane refilling its buffer at
-// its own moment waits for LDS only. While the state is at least 2^23 before
-// every symbol, a symbol leaves it at least 2^11 and renormalising takes at
-// most 2 bytes, so a ring holding two intervals never runs dry. A stream that
-// breaks this (an initial state below 2^23, or one that ends before its
-// symbols do: corrupt input only) marks its lane, which then decodes the
-// brick again by the sequential rule, byte by byte, so every stream decodes
-// exactly as that rule reads it.
+// from a ring word read ahead of use, once per pair of symbols. Global memory
+// is touched only every LR_CHECK symbols, an iteration all lanes of the wave
+// share: the blocks loaded at the previous checkpoint go into the ring, the
+// next ones are requested, and finished nibble words are stored. Waits for
+// global memory (which complete in issue order, per wave) therefore only ever
+// cover operations issued a whole interval earlier; a lane refilling its
+// buffer at its own moment waits for LDS only. While the state is at least
+// 2^23 before every symbol, a symbol leaves it at least 2^11 and
+// renormalising takes at most 2 bytes, so a lane reads at most 8 ring words
+// per interval and a ring holding two intervals never runs dry. A stream that
+// breaks this (an initial state below 2^23, a state of 2^31 or more after a
+// symbol, or a stream that ends before its symbols do: corrupt input only),
+// or a read index found past the write index, marks its lane, which then
+// decodes the brick again by the sequential rule, byte by byte, so every
+// stream decodes exactly as that rule reads it.
 constexpr uint32_t LANE_NT = 256;
 constexpr uint32_t LR_BLOCKS = 4;             // 64 bytes per lane
 constexpr uint32_t LR_WORDS = LR_BLOCKS * 4;
@@ -425,8 +427,11 @@ struct LaneStream {
     prefetch();
   }
   // every lane of the wave, the same symbol: stage the blocks requested one
-  // interval ago, request the next ones
+  // interval ago, request the next ones. A read index past the write index
+  // (a refill took a ring word not yet written) marks the lane for the
+  // sequential redo.
   __device__ __forceinline__ void checkpoint() {
+    bad = bad || (int32_t)(wrx - rdx) < 0;
     if (staged && wrx - rdx + 8u * LR_ROW <= LR_WORDS * LR_ROW) {
       put(s0);
       put(s1);
@@ -463,8 +468,9 @@ struct LaneStream {
   __device__ __forceinline__ void top_up() {
     const bool low = nbits < T;
     bb |= (uint64_t)(low ? pre : 0u) << ((32u - nbits) & 63u);
-    nbits += low ? 32u : 0u;
-    rdx += low ? LR_ROW : 0u;
+    const uint32_t r = low ? 32u : 0u;
+    nbits += r;
+    rdx += r * (LR_ROW / 32u);
   }
 };
 
@@ -597,6 +603,7 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
     if ((e & 15u) != 0) status |= FCS_ST_RANS_STATE;
   }
   if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
+  rd.bad = rd.bad || (int32_t)(rd.wrx - rd.rdx) < 0;
   if (rd.bad) status = lane_redo(a.stream + B.str_off, E, nsym, nibw, table);
   a.res[b] = RansResult{status, 0u};
 }

@@ -220,8 +220,8 @@ def test_ops_split_autotune_512cube_b32(gpu_ctx, cfg2):
 def test_rans_collapsed_state_reads_past_two_bytes(gpu_ctx, mapping):
     """A stream whose final state bytes are zeroed starts from state 0: the
     first symbols renormalise with more than the two bytes a valid stream can
-    need per symbol (the lane decoder's direct-read path). Nibbles and
-    statuses still equal the oracle's sequential decode."""
+    need per symbol (the lane decoder redoes such a brick by the sequential
+    rule). Nibbles and statuses still equal the oracle's sequential decode."""
     print(f"[gpu] collapsed-state streams, mapping {mapping}", flush=True)
     data = bytearray(golden_bytes("cfg1_b32"))
     ds0 = O.Dataset(bytes(data), 32)
@@ -245,6 +245,38 @@ def test_rans_collapsed_state_reads_past_two_bytes(gpu_ctx, mapping):
         vol.close()
     finally:
         gpu_ctx.set_rans_mapping(0)
+
+@pytest.mark.parametrize("mapping", [1, 2, 5])
+def test_rans_initial_state_above_2_31(gpu_ctx, mapping):
+    """A stream whose last byte (the top byte of the initial state) is 0xFF
+    starts from a state of at least 2^31, which no valid stream reaches; the
+    first symbols then leave states that need no renormalisation or wrap the
+    lane decoder's byte count, so the lane decodes the brick again by the
+    sequential rule. Nibbles and statuses equal the oracle's decode."""
+    print(f"[gpu] initial state >= 2^31, mapping {mapping}", flush=True)
+    data = bytearray(golden_bytes("cfg1_b32"))
+    ds0 = O.Dataset(bytes(data), 32)
+    nb = ds0.num_bricks
+    hit = [i for i in range(nb) if ds0.brick(i).E > 16][3::5]
+    assert hit
+    for i in hit:
+        br = ds0.brick(i)
+        data[br.str_off + br.E - 1] = 0xFF
+    ds = O.Dataset(bytes(data), 32)
+    gpu_ctx.set_rans_mapping(mapping)
+    try:
+        vol = gpu_ctx.load(bytes(data), 32)
+        st = vol.decode_rans(raise_on_error=False)
+        for i in range(nb):
+            want, wst = ds.rans(i)
+            got = vol.read_nibbles(i)
+            assert np.array_equal(got, want[:got.size]) and got.size == (ds.brick(i).nsym + 1) // 2, i
+            assert (st[i] != 0) == (wst != 0), (i, st[i], wst)
+        assert all(st[i] != 0 for i in hit)
+        vol.close()
+    finally:
+        gpu_ctx.set_rans_mapping(0)
+
 
 @pytest.mark.parametrize("mapping", [1, 2, 3, 4, 5, 6])
 def test_rans_nibbles_equal_oracle_with_corrupt_bricks(gpu_ctx, manifest, mapping):
