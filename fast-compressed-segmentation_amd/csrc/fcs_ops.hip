@@ -809,7 +809,7 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   auto act_node = [&](uint32_t k) -> uint32_t {
     if constexpr (C::PACKED) return S.act32[k >> 3] & 0xFFFFu; else return S.act16[k >> 3];
   };
-  uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0;
+  uint32_t cnt = 0, inc = 0, w0 = 0, w1 = 0, mine = 0;
   const uint32_t nw = (Cn + 31) >> 5;
   if (mode == MODE_LEVEL) {
     // one active parent's 8 children per step: one 16-byte LDS read, the
@@ -839,12 +839,24 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
         *reinterpret_cast<uint4*>(vnext + n0) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
-    // active nodes of level l+1, over consecutive stop words per thread
-    const uint32_t wpt = (nw + NT - 1) / NT;
-    w0 = min(nw, tid * wpt);
-    w1 = min(nw, w0 + wpt);
+    // active nodes of level l+1: each thread takes a span of Cn / NT
+    // consecutive nodes (within one stop word; whole words per thread when a
+    // level has more than 32 nodes per thread)
     const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
-    for (uint32_t w = w0; w < w1; ++w) cnt += __popc(~snext[w] & valid);
+    const uint32_t span = Cn >= NT ? Cn / NT : 1u;
+    if (span <= 32u) {
+      const uint32_t bit0 = tid * span;
+      if (bit0 < Cn) {
+        const uint32_t sw = ~snext[bit0 >> 5] & valid;
+        mine = span == 32u ? sw : (sw >> (bit0 & 31u)) & ((1u << span) - 1u);
+      }
+      cnt = __popc(mine);
+    } else {
+      const uint32_t wpt = (nw + NT - 1) / NT;
+      w0 = min(nw, tid * wpt);
+      w1 = min(nw, w0 + wpt);
+      for (uint32_t w = w0; w < w1; ++w) cnt += __popc(~snext[w] & valid);
+    }
     inc = wave_incl_scan(cnt);
     if (lane == 63) S.scr[SCR_WAVESUM + wave] = inc;
   } else if (mode == MODE_COMPACT) {
@@ -932,15 +944,30 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
     } else {
       uint32_t k0 = wb + inc - cnt;
       const uint32_t valid = (Cn >= 32) ? 0xFFFFFFFFu : ((1u << Cn) - 1u);
-      for (uint32_t w = w0; w < w1; ++w) {
-        S.wpre[w] = (uint16_t)k0;
-        uint32_t bits = ~snext[w] & valid;
+      const uint32_t span = Cn >= NT ? Cn / NT : 1u;
+      if (span <= 32u) {
+        // the thread whose span opens a stop word records the count before it
+        const uint32_t bit0 = tid * span;
+        if (bit0 < Cn && (bit0 & 31u) == 0u) S.wpre[bit0 >> 5] = (uint16_t)k0;
+        uint32_t bits = mine;
         while (bits) {
           const uint32_t j = __ffs(bits) - 1;
           bits &= bits - 1;
-          const uint32_t m = w * 32 + j;
+          const uint32_t m = bit0 + j;
           if constexpr (C::PACKED) S.act32[k0++] = m | ((uint32_t)vnext[m] << 16);
           else S.act16[k0++] = (uint16_t)m;
+        }
+      } else {
+        for (uint32_t w = w0; w < w1; ++w) {
+          S.wpre[w] = (uint16_t)k0;
+          uint32_t bits = ~snext[w] & valid;
+          while (bits) {
+            const uint32_t j = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const uint32_t m = w * 32 + j;
+            if constexpr (C::PACKED) S.act32[k0++] = m | ((uint32_t)vnext[m] << 16);
+            else S.act16[k0++] = (uint16_t)m;
+          }
         }
       }
       if (l + 2 < t) {
