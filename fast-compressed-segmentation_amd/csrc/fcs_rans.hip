@@ -344,15 +344,17 @@ __global__ __launch_bounds__(64) void rans_win_kernel(RansArgs a) {
 // next ones are requested, and finished nibble words are stored. Waits for
 // global memory (which complete in issue order, per wave) therefore only ever
 // cover operations issued a whole interval earlier; a lane refilling its
-// buffer at its own moment waits for LDS only. While the state is at least
-// 2^23 before every symbol, a symbol leaves it at least 2^11 and
-// renormalising takes at most 2 bytes, so a lane reads at most 8 ring words
-// per interval and a ring holding two intervals never runs dry. A stream that
-// breaks this (an initial state below 2^23, a state of 2^31 or more after a
-// symbol, or a stream that ends before its symbols do: corrupt input only),
-// or a read index found past the write index, marks its lane, which then
-// decodes the brick again by the sequential rule, byte by byte, so every
-// stream decodes exactly as that rule reads it.
+// buffer at its own moment waits for LDS only. From an initial state in
+// [2^23, 2^31), every symbol leaves the state in [2^11, 2^31) and
+// renormalising brings it back to [2^23, 2^31) with at most 2 bytes, so a
+// lane reads at most 8 ring words per interval and a ring holding two
+// intervals never runs dry; the bytes left need no per-symbol count, since
+// the sequential rule's "and bytes are left" only differs once the stream has
+// been read past its start, which the bits consumed show at the end. A lane
+// whose initial state is outside that range, whose stream was read past its
+// start, or whose read index was found past the write index (corrupt input
+// only) decodes its brick again by the sequential rule, byte by byte, so
+// every stream decodes exactly as that rule reads it.
 constexpr uint32_t LANE_NT = 256;
 constexpr uint32_t LR_BLOCKS = 4;             // 64 bytes per lane
 constexpr uint32_t LR_WORDS = LR_BLOCKS * 4;
@@ -369,6 +371,7 @@ struct LaneStream {
   // rd never passes wr and they differ by at most LR_WORDS words, so
   // comparisons survive wrapping (a bad lane's values are discarded)
   uint32_t rdx, wrx;
+  uint32_t rdx0, bits0;  // read index and stream bits in bb after the initial fill
   uint64_t bb;        // upcoming stream bytes, the next one in bits 63..56
   uint32_t nbits;     // valid bits in bb (a multiple of 8)
   uint32_t pre;       // ring word rd, read at the end of the previous symbol
@@ -424,6 +427,8 @@ struct LaneStream {
     bb |= (uint64_t)at(rdx) << (32u - nbits);
     rdx += LR_ROW;
     nbits += 32u;  // 40..64 bits: the initial state and more
+    rdx0 = rdx;
+    bits0 = nbits;
     prefetch();
   }
   // every lane of the wave, the same symbol: stage the blocks requested one
@@ -445,21 +450,18 @@ struct LaneStream {
     }
   }
   // renormalise after a symbol: while state < 2^23, shift the next byte in.
-  // A state in [2^11, 2^31) takes (clz(state) - 1) rounded down to a multiple
-  // of 8 bits: 0, 1 or 2 bytes, shifted in from bb at once (bb holds at least
-  // 16 bits here). A state outside that range or fewer bytes left than taken
-  // happen on corrupt input only and mark the lane for the sequential redo.
-  // A lane not marked keeps its state at least 2^23 before each symbol, so a
-  // symbol leaves it in [2^11, 2^31).
-  __device__ __forceinline__ void shift(uint32_t& state, uint32_t& pos) {
+  // A state in [2^11, 2^31) (every state of a lane whose initial state is in
+  // [2^23, 2^31)) takes (clz(state) - 1) rounded down to a multiple of 8
+  // bits: 0, 1 or 2 bytes, shifted in from bb at once (bb holds at least 16
+  // bits here). Other lanes compute garbage here and are redone.
+  __device__ __forceinline__ void shift(uint32_t& state) {
     const uint32_t sh = ((uint32_t)__clz((int)state) - 1u) & ~7u;
-    const uint32_t n = sh >> 3;
-    bad = bad || n > pos;  // also a state of 2^31 or more (sh wraps)
-    pos -= n;
     state = (uint32_t)(((((uint64_t)state) << 32) | (bb >> 32)) >> ((32u - sh) & 63u));
     bb <<= sh & 63u;
     nbits -= sh;
   }
+  // stream bits consumed so far (the bits taken into bb less those still in it)
+  __device__ __forceinline__ uint32_t consumed() const { return bits0 + 32u * ((rdx - rdx0) / LR_ROW) - nbits; }
   // fewer than T bits left (T <= 32): append the word prefetched from the
   // ring below them. With at most 2 bytes per symbol and a refill of one
   // word, a lane not marked reads at most 8 ring words per 16 symbols, which
@@ -529,18 +531,18 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   rd.ring = reinterpret_cast<char*>(rings + threadIdx.x);
   rd.init(a.stream + B.str_off, E);
   // the initial state: the stream's last 4 bytes
-  uint32_t pos = E - 4;
   uint32_t state = (uint32_t)(rd.bb >> 32);
   rd.bb <<= 32;
   rd.nbits -= 32u;
   // at least 32 bits for the first pair of symbols
   if (rd.nbits < 32u) rd.refill();
   rd.prefetch();
-  rd.bad = rd.bad || state < FCS_RANS_L;
+  // a state in [2^23, 2^31) before every symbol keeps every later one there
+  rd.bad = rd.bad || state < FCS_RANS_L || state >= 0x80000000u;
   // one symbol; at least 16 bits left for the next
   auto step = [&]() -> uint32_t {
     const uint32_t e = lane_step(state, table);
-    rd.shift(state, pos);
+    rd.shift(state);
     rd.top_up<16u>();
     rd.prefetch();
     return e;
@@ -549,9 +551,9 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   // to at least 32 bits for the next pair
   auto step2 = [&](uint32_t& word) {
     const uint32_t e0 = lane_step(state, table);
-    rd.shift(state, pos);
+    rd.shift(state);
     const uint32_t e1 = lane_step(state, table);
-    rd.shift(state, pos);
+    rd.shift(state);
     rd.top_up<32u>();
     rd.prefetch();
     word = (word >> 4) | (e0 << 28);
@@ -599,11 +601,13 @@ __global__ __launch_bounds__(LANE_NT) void rans_lane_kernel(RansArgs a) {
   uint32_t status = 0;
   if (nsym & 1u) {
     const uint32_t e = lane_step(state, table);
-    rd.shift(state, pos);
+    rd.shift(state);
     if ((e & 15u) != 0) status |= FCS_ST_RANS_STATE;
   }
-  if (state != FCS_RANS_L || pos != 0) status |= FCS_ST_RANS_STATE;
-  rd.bad = rd.bad || (int32_t)(rd.wrx - rd.rdx) < 0;
+  // a stream that ran out before its symbols did was read past its start
+  const uint32_t used = rd.consumed(), have = 8u * E;
+  if (state != FCS_RANS_L || used != have) status |= FCS_ST_RANS_STATE;
+  rd.bad = rd.bad || used > have || (int32_t)(rd.wrx - rd.rdx) < 0;
   if (rd.bad) status = lane_redo(a.stream + B.str_off, E, nsym, nibw, table);
   a.res[b] = RansResult{status, 0u};
 }
