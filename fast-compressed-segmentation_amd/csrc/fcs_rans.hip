@@ -455,9 +455,9 @@ struct LaneStream {
   // bits: 0, 1 or 2 bytes, shifted in from bb at once (bb holds at least 16
   // bits here). Other lanes compute garbage here and are redone.
   __device__ __forceinline__ void shift(uint32_t& state) {
-    const uint32_t sh = ((uint32_t)__clz((int)state) - 1u) & ~7u;
-    state = (uint32_t)(((((uint64_t)state) << 32) | (bb >> 32)) >> ((32u - sh) & 63u));
-    bb <<= sh & 63u;
+    const uint32_t sh = ((uint32_t)__clz((int)state) - 1u) & 24u;  // 0, 8 or 16 here (24 only on a redone lane)
+    state = (uint32_t)(((((uint64_t)state) << 32) | (bb >> 32)) >> (32u - sh));
+    bb <<= sh;
     nbits -= sh;
   }
   // stream bits consumed so far (the bits taken into bb less those still in it)
