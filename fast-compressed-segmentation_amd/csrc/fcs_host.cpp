@@ -274,7 +274,7 @@ int stage_decode(fcs_volume* v, DecodeLane& ln, const std::vector<DecodeReq>& re
   if (n == 0) return 0;
   if (int rc = ensure_req(v, ln, n)) return rc;
   // Bin the requests by estimated cost, heaviest first (stable within a bin):
-  // the persistent decoder deals the front half round-robin and claims the
+  // the persistent decoder deals the front three quarters round-robin and claims the
   // rest, so light requests fill the end. Cost ~ nibbles up to the LOD plus
   // labels written; bins are eighths of an octave, so requests of about the
   // same cost are claimed together (octave bins alone left each octave in

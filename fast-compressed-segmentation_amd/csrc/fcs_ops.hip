@@ -1302,7 +1302,7 @@ __global__ __launch_bounds__(64) void ops_head_kernel(OpsArgs<LT> a) {
 }
 
 // Persistent workgroups drain the request list, which the host orders by
-// estimated cost, heaviest first. Claims are numbered: the first half is dealt
+// estimated cost, heaviest first. Claims are numbered: the first three quarters are dealt
 // round-robin (equal shares, since neighbouring requests cost about the same);
 // the rest is claimed through one atomic counter to even out the finishing
 // times. (One counter shared by every workgroup serialises: claiming every
@@ -1329,7 +1329,10 @@ __global__ __launch_bounds__(C::NT, C::MINW) void ops_kernel(OpsArgs<LT> a) {
   const DecodeReq none{0xFFFFFFFFu, 0u, 0ull};
   const DevBrick nobrick{0ull, 0ull, 0u, 0u, 0u, 0u};
   const uint32_t G = gridDim.x;
-  const uint32_t n_static = nreq <= G ? nreq : max((nreq / 2) / G * G, G);
+  // the first three quarters of the claims are dealt round-robin (dealing a
+  // half measured 1 % slower, dealing all 5 % slower: the claimed tail
+  // evens out the workgroups' ends)
+  const uint32_t n_static = nreq <= G ? nreq : max((nreq / 4 * 3) / G * G, G);
   uint32_t sj = 0;  // thread 0: round-robin requests taken so far
   // claim c -> request: chunk c / G even takes the next G requests from the
   // front, odd from the back (a bijection on [0, nreq))
