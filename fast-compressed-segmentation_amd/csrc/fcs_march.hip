@@ -189,12 +189,13 @@ __device__ float shadow_ray(const MarchArgs<LT>& a, const float start[3], const 
 }
 
 template <typename LT, bool RGBA, bool STATS>
-__global__ __launch_bounds__(256) void march_kernel(MarchArgs<LT> a) {
-  // a workgroup's 16x16 pixels as four 8x8 tiles, one per wave: a wave's rays
-  // start closer together than in a 16x4 strip and run more alike
+__global__ __launch_bounds__(128) void march_kernel(MarchArgs<LT> a) {
+  // a workgroup's 16x8 pixels as two 8x8 tiles side by side, one per wave: a
+  // wave's rays start closer together than in a 16x4 strip and run more alike;
+  // two-wave workgroups finish a frame's ragged tail sooner than four-wave ones
   const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
-  const uint32_t px = blockIdx.x * 16 + (wv & 1u) * 8 + (ln & 7u);
-  const uint32_t py = blockIdx.y * 16 + (wv >> 1) * 8 + (ln >> 3);
+  const uint32_t px = blockIdx.x * 16 + wv * 8 + (ln & 7u);
+  const uint32_t py = blockIdx.y * 8 + (ln >> 3);
   if (px >= a.w || py >= a.h) return;
   if (a.run_if && *a.run_if == 0u) return;
   const size_t pix = (size_t)py * a.w + px;
@@ -747,14 +748,14 @@ hipError_t fcs_launch_zero_if(uint32_t* p, uint64_t n, const uint32_t* flag, hip
 
 template <typename LT>
 hipError_t fcs_launch_march(const MarchArgs<LT>& a, hipStream_t s) {
-  const dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
+  const dim3 grid((a.w + 15) / 16, (a.h + 7) / 8);
   if (a.out_rgba) {
     if (!a.tf_keys || !a.tf_rgba) return hipErrorInvalidValue;
-    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, true, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((march_kernel<LT, true, false>), grid, dim3(256), 0, s, a);
+    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, true, true>), grid, dim3(128), 0, s, a);
+    else hipLaunchKernelGGL((march_kernel<LT, true, false>), grid, dim3(128), 0, s, a);
   } else {
-    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, false, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((march_kernel<LT, false, false>), grid, dim3(256), 0, s, a);
+    if (a.stats) hipLaunchKernelGGL((march_kernel<LT, false, true>), grid, dim3(128), 0, s, a);
+    else hipLaunchKernelGGL((march_kernel<LT, false, false>), grid, dim3(128), 0, s, a);
   }
   return hipGetLastError();
 }
