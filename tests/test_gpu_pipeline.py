@@ -78,7 +78,9 @@ def test_decode_volume_forced_mappings(ctx, manifest, name, mapping):
 
 def test_decode_volume_corrupt_streams(ctx):
     """Damaged streams: per-brick statuses are the rANS stage's bits or'ed with
-    the op decoder's, and every brick's labels equal the two-call path's."""
+    the op decoder's, and every brick's labels equal the two-call path's; a
+    second call (which launches the requests the first one staged) reports
+    the same statuses and labels."""
     print("[gpu] decode_volume corrupt streams", flush=True)
     data = bytearray(golden_bytes("rand_b8"))
     ds = O.Dataset(bytes(data), 8)
@@ -92,6 +94,9 @@ def test_decode_volume_corrupt_streams(ctx):
     vol = ctx.load(data, 8)
     st = vol.decode_volume(L, False, raise_on_error=False)
     got = vol.read_cache(0, n * nb).reshape(nb, n).copy()
+    st2 = vol.decode_volume(L, False, raise_on_error=False)
+    assert np.array_equal(st2, st)
+    assert np.array_equal(vol.read_cache(0, n * nb).reshape(nb, n), got)
     vol.close()
     vol = ctx.load(data, 8)
     rst = vol.decode_rans(raise_on_error=False)
@@ -158,8 +163,11 @@ def test_decode_volume_repeated_calls_and_residency_changes(ctx):
     """Repeated calls, and calls after a residency change in between --
     requests at another LOD, another LOD for the whole volume, a
     direct-mapped pool, a device-built frame that re-decodes bricks at their
-    frame LOD: every call leaves every brick's finest labels equal to the
-    oracle's and every brick resident at that LOD."""
+    frame LOD, a full decode through decode_all, a pyramid-layout call: every
+    call leaves every brick's finest labels equal to the oracle's and every
+    brick resident at that LOD. (A call launches the requests the previous
+    one staged when nothing restaged them and the LOD, layout and slot size
+    are the same; the calls in between restage them or change the key.)"""
     print("[gpu] decode_volume repeated calls", flush=True)
     data = golden_bytes("cfg1_b32")
     ds = O.Dataset(data, 32)
@@ -188,6 +196,14 @@ def test_decode_volume_repeated_calls_and_residency_changes(ctx):
         check(per)
         cam = vol.orbit_camera(0.7, 0.3, 3.0 * vol.info["grid_size"], math.pi / 3)
         vol.render(cam, 160, 120, want=())  # device-built frame: bricks re-decoded at their frame LOD
+        check(per)
+        vol.decode_all(L, True)  # restages the context's requests
+        check(per)
+        assert np.all(vol.decode_volume(L, True) == 0)  # pyramid layout: other slots
+        pyr = O.pyramid_offset(L + 1)
+        got = np.stack([vol.read_cache(i * pyr + O.pyramid_offset(L), per) for i in range(nb)]).astype(np.uint64)
+        assert np.array_equal(got, want)
+        check(per)
         check(per)
     finally:
         vol.close()
