@@ -136,6 +136,10 @@ struct fcs_volume {
   // calls), `lane_aux` on the aux stream (prefetches, finished later)
   DecodeLane lane_main, lane_aux;
   uint64_t stage_gen = 0;  // source of DecodeLane::gen
+  // pinned host copies of the per-brick results read back after a full
+  // decode (num_bricks entries each; allocated at first use)
+  RansResult* h_rres = nullptr;
+  uint32_t* h_vstatus = nullptr;
   // fcs_decode_volume's binned requests as staged in lane_main: reused while
   // the lane still holds them (same generation) for the same LOD, layout and
   // cache slot size
@@ -982,6 +986,10 @@ void fcs_volume_release_device(fcs_volume* v) {
     if (q) (void)hipFree(q);
   free_lane(v->lane_main);
   free_lane(v->lane_aux);
+  if (v->h_rres) (void)hipHostFree(v->h_rres);
+  if (v->h_vstatus) (void)hipHostFree(v->h_vstatus);
+  v->h_rres = nullptr;
+  v->h_vstatus = nullptr;
   auto& vs = c->volumes;
   vs.erase(std::remove(vs.begin(), vs.end(), v), vs.end());
   v->ctx = nullptr;
@@ -1035,8 +1043,10 @@ RansArgs rans_args(const fcs_volume* v) {
 // must have every rANS launch queued) into the volume; true if any failed.
 int settle_rans(fcs_volume* v, uint32_t* out_status, bool* bad) {
   fcs_ctx* c = v->ctx;
-  std::vector<RansResult> rr(v->nb);
-  HIP_TRY(c, hipMemcpyAsync(rr.data(), v->d_rres, rr.size() * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
+  if (!v->h_rres)
+    HIP_TRY(c, hipHostMalloc((void**)&v->h_rres, std::max<size_t>(v->nb, 1) * sizeof(RansResult), hipHostMallocDefault));
+  const RansResult* rr = v->h_rres;
+  HIP_TRY(c, hipMemcpyAsync(v->h_rres, v->d_rres, (size_t)v->nb * sizeof(RansResult), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   *bad = false;
   for (uint32_t i = 0; i < v->nb; ++i) {
@@ -1342,6 +1352,8 @@ static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t
     }
     mode = v->rans_tuned;
   }
+  if (!v->h_vstatus)
+    HIP_TRY(c, hipHostMalloc((void**)&v->h_vstatus, std::max<size_t>(v->nb, 1) * sizeof(uint32_t), hipHostMallocDefault));
   // The rANS stage needs no requests, so it goes first and the host builds
   // the requests while it runs.
   HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
@@ -1376,8 +1388,8 @@ static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t
   HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   account_decode(v, reqs);  // host bookkeeping while the device decodes
   if (!ln.reqs.empty())
-    HIP_TRY(c, hipMemcpyAsync(ln.status_sorted.data(), ln.d_status, ln.reqs.size() * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(v->h_vstatus, ln.d_status, ln.reqs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
   bool rbad = false;
   if (int rc = settle_rans(v, nullptr, &rbad)) return rc;  // synchronises the context stream
   float ms = 0;
@@ -1390,7 +1402,7 @@ static int decode_volume_body(fcs_volume* v, uint32_t lod, int pyramid, uint32_t
   const size_t n = ln.reqs.size();
   for (size_t j = 0; j < n; ++j) {
     const uint32_t b = v->rlist[ln.perm[j]];
-    const uint32_t st = v->rans_status[b] | ln.status_sorted[j];
+    const uint32_t st = v->rans_status[b] | v->h_vstatus[j];
     if (out_status) out_status[b] = st;
     if (st) {
       ++nbad;
