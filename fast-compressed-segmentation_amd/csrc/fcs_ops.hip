@@ -744,10 +744,12 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
   constexpr bool ONE = NW == 1;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const uint32_t t = c.t, P = c.P, nsym = c.nsym;
-  // A level step's waves issue after the label-writing waves of the CU's
-  // other workgroups (wave priority 0 here, 2 while labels are written):
-  // keeping the label stores flowing took the 2048^3 decode call from
-  // 10.88-10.91 to 10.78-10.80 ms; the reverse priorities measured 11.02.
+  // Wave priorities among the CU's workgroups rise toward the label stores:
+  // 0 for the parse, 1 from the reference chase on, 2 while labels are
+  // written, so the stores keep flowing and a workgroup near its labels goes
+  // ahead of one that is parsing. On the 2048^3 decode call: 10.88-10.91 ms
+  // with one priority, 10.78-10.80 with 0 / 2, 10.68 with 0 / 1 / 2; the
+  // reverse (level steps above labels) measured 11.02.
   if constexpr (!ONE) __builtin_amdgcn_s_setprio(0);
   const uint32_t N = 1u << (3 * l), Cn = N << 3, K = 8 * v.A;
   const bool last = (l + 1 == t);
@@ -808,6 +810,7 @@ __device__ __forceinline__ bool level_step(const OpsArgs<LT>& a, const Lds<LT>& 
     }
   }
 
+  if constexpr (!ONE) __builtin_amdgcn_s_setprio(1);
   // ---- P2: chase references. A valid reference points to a smaller node,
   // so a chain strictly decreases; anything else is corrupt input. Only op
   // children can hold references.
